@@ -1,0 +1,152 @@
+/* parsec_hip.h -- C-ABI of libparsec_hip.so, the MI355X (gfx950) kernels of the
+ * PARSEC.py SCF hot path.
+ *
+ * Conventions
+ *   - Every entry point returns int: 0 = success, 1 = bad argument,
+ *     2 = HIP runtime error, 3 = dense solver failure, 4 = size limit.
+ *     pc_last_error() returns the text of the last failure (per host thread).
+ *   - Device operands are plain device pointers owned by the caller; `stream`
+ *     is a hipStream_t passed as void*.  Calls are asynchronous on that stream
+ *     unless stated otherwise.
+ *   - A "block" of k vectors on the domain is vector-major: element (p, v) at
+ *     v*ld + p, p < npts (the reference's N x k array W in column-major order).
+ *   - The domain is a set of x-rows of an nx*ny*nz grid, row (j,k) holding the
+ *     contiguous points row_lo <= i < row_hi; points are numbered k-major, then
+ *     j, then i (the reference's flat order i + nx*j + nx*ny*k restricted to the
+ *     domain).
+ *
+ * Each declaration names the reference interface (zeyizhang-berkeley/
+ * parsec_python, path:line) whose work it replaces.
+ */
+#ifndef PARSEC_HIP_H
+#define PARSEC_HIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+const char* pc_last_error(void);
+int pc_version(void);
+
+/* ---- grid and the fused stencil -------------------------------------------
+ * replaces: src/Laplacian/fd3d.py:34-106 (fd3d) and every `H @ X` SpMM of
+ * src/Eigensolvers/ch_filter.py:66-80, first_filt.py:94-101, chefsi1.py:46,
+ * lanczosForChefsi1.py:41 (CPU) / chefsi1_gpu.py:51-57 (CuPy CSR SpMM).
+ * weights[0..radius]: 1D weights of -d2/dx2 (centre first), radius in {1,2,3,4,6}.
+ * out = sum_d scale*w_d*(6 neighbours) + 3*scale*w_0*y + (a*V + b)*y + c*x.
+ * x may alias out; y must not.  dot_part (optional) receives per-workgroup
+ * sums of out*y (see pc_stencil_num_partials). */
+int pc_grid_create(int nx, int ny, int nz, const int* row_lo, const int* row_hi, int radius,
+                   const double* weights, void** out_grid, long long* out_npts);
+int pc_grid_destroy(void* grid);
+int pc_stencil_f64(void* grid, const double* Y, const double* X, double* Out, const double* V, int nvec,
+                   long long ld, double scale, double a, double b, double c, double* dot_part, void* stream);
+int pc_stencil_f32(void* grid, const float* Y, const float* X, float* Out, const float* V, int nvec,
+                   long long ld, double scale, double a, double b, double c, double* dot_part, void* stream);
+int pc_stencil_direct_f64(void* grid, const double* Y, const double* X, double* Out, const double* V, int nvec,
+                          long long ld, double scale, double a, double b, double c, double* dot_part, void* stream);
+int pc_stencil_direct_f32(void* grid, const float* Y, const float* X, float* Out, const float* V, int nvec,
+                          long long ld, double scale, double a, double b, double c, double* dot_part, void* stream);
+int pc_stencil_num_partials(void* grid, int dtype_bytes, int nvec, int* out_count, int* out_per_vector);
+
+/* ---- Kleinman-Bylander projectors (factored) --------------------------------
+ * replaces: src/V_ion/pseudoNL_original.py:9-208 (V_nl as dense outer
+ * products in a sparse matrix, applied inside every SpMM).
+ * atom_ptr[natoms+1] groups entries; ent_pt[nent] point indices; ent_u[3*nent]
+ * (u_x,u_y,u_z); scale[natoms] = h^3/Zvalue.  C is FP64 (3*natoms) x nvec. */
+int pc_proj_create(int natoms, const int* atom_ptr, const int* ent_pt, const double* ent_u, const double* scale,
+                   long long npts, void** out_proj);
+int pc_proj_destroy(void* proj);
+int pc_proj_ncols(void* proj, int* out_ncols);
+int pc_proj_gather_f64(void* proj, const double* Y, long long ld, int nvec, double* C, void* stream);
+int pc_proj_gather_f32(void* proj, const float* Y, long long ld, int nvec, double* C, void* stream);
+int pc_proj_scatter_f64(void* proj, const double* C, double* Out, long long ld, int nvec, double alpha,
+                        void* stream);
+int pc_proj_scatter_f32(void* proj, const double* C, float* Out, long long ld, int nvec, double alpha,
+                        void* stream);
+
+/* ---- Rayleigh-Ritz GEMMs on MFMA -------------------------------------------
+ * replaces: the vdot double loop src/Eigensolvers/chefsi1.py:51-53,
+ * first_filt.py:53-57 (CuPy: W.T @ Vin, chefsi1_gpu.py:61) and W @ Q
+ * (chefsi1.py:67, chefsi1_gpu.py:66).
+ * gram: S (kx x ky, column-major, lds) = X^T Y in FP64 (f32 inputs: FP32 MFMA
+ * per chunk, FP64 across chunks); sym=1 computes the upper tiles of X^T X.
+ * rotate: Z = X Q, Q FP64 kx x ky column-major (ldq); work >= kx*ky*sizeof(T). */
+int pc_gram_workspace(long long n, int kx, int ky, int sym, size_t* out_bytes);
+int pc_gram_f64(const double* X, long long ldx, const double* Y, long long ldy, long long n, int kx, int ky,
+                double* S, long long lds, int sym, void* work, size_t work_bytes, void* stream);
+int pc_gram_f32(const float* X, long long ldx, const float* Y, long long ldy, long long n, int kx, int ky,
+                double* S, long long lds, int sym, void* work, size_t work_bytes, void* stream);
+int pc_rotate_f64(const double* X, long long ldx, const double* Q, int ldq, double* Z, long long ldz, long long n,
+                  int kx, int ky, void* work, size_t work_bytes, void* stream);
+int pc_rotate_f32(const float* X, long long ldx, const double* Q, int ldq, float* Z, long long ldz, long long n,
+                  int kx, int ky, void* work, size_t work_bytes, void* stream);
+
+/* ---- FP64 dense k x k stage on the GPU (rocSOLVER/rocBLAS) ------------------
+ * replaces: scipy.linalg.qr + eigh in src/Eigensolvers/chefsi1.py:43,64 and
+ * first_filt.py:43,59 (cuSOLVER qr/eigh in chefsi1_gpu.py:54,65).
+ * pc_dense_rr: S = R^T R (potrf), G = R^-T Gy R^-1, G = Q L Q^T (syevd),
+ * T = R^-1 Q; lam_host receives the ascending Ritz values.  Synchronises. */
+int pc_dense_create(void* stream, int kmax, void** out_ctx);
+int pc_dense_destroy(void* ctx);
+int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, double* T, int ldt, double* lam_dev,
+                double* lam_host);
+int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
+
+/* ---- point-wise SCF kernels --------------------------------------------------
+ * density: rho = sum_v occ2[v] W_v^2           (src/rsdft_solver.py:324)
+ * scf_post: LDA PZ81 v_xc, pot_new = ppot + 0.5 vxc + hpot + hpot0 and the
+ *   sums [E_xc/h^3, |pot_new-pot|^2, |pot_new|^2, sum rho(hpot+hpot0),
+ *   sum rho vxc]                                (src/V_xc/exc_nspn.py:8-103;
+ *   src/rsdft_solver.py:128-153,350-352)
+ * hartree_rhs: rhs = fac (rho - rho0)            (src/rsdft_solver.py:327)
+ * dots / lincomb: the multisecant mixer's GEMV work (src/Mixer/msecant1.py:104-167)
+ * rand: counter-based (splitmix64) uniform / normal start blocks
+ *   (replaces np.random.rand/randn, src/Eigensolvers/first_filt.py:37,
+ *   lancz_uppbnd.py:21, chefsi1.py:29). */
+int pc_density_f64(const double* W, long long ld, long long n, int nev, const double* occ2, double* rho,
+                   void* stream);
+int pc_density_f32(const float* W, long long ld, long long n, int nev, const double* occ2, double* rho,
+                   void* stream);
+int pc_num_blocks(long long n, int* out);
+int pc_scf_post(long long n, const double* rho_cell, double h3, const double* ppot, const double* hpot,
+                const double* hpot0, const double* pot_old, double* rho_out, double* vxc_out, double* pot_new,
+                double* part, double* sums5, void* stream);
+int pc_xc(long long n, const double* rho, double* vxc, double* part, double* exc_sum, void* stream);
+int pc_hartree_rhs(long long n, const double* rho_cell, const double* rho0, double fac, double* rhs, void* stream);
+int pc_reduce_sets(const double* part, int nblk, int nsets, double* out, void* stream);
+int pc_dots_nchunk(long long n, int* out);
+int pc_dots(long long n, const double* A, long long lda, int m, const double* v, double* part, double* out,
+            void* stream);
+int pc_lincomb(long long n, const double* A, long long lda, int m, const double* coef_host, double alpha,
+               const double* x, double beta, double* y, void* stream);
+int pc_convert_f64_to_f32(long long n, int ncol, const double* src, long long lds, float* dst, long long ldd,
+                          void* stream);
+int pc_convert_f32_to_f64(long long n, int ncol, const float* src, long long lds, double* dst, long long ldd,
+                          void* stream);
+int pc_rand_f64(long long n, int ncol, long long ld, unsigned long long seed, int normal, double* out, void* stream);
+int pc_rand_f32(long long n, int ncol, long long ld, unsigned long long seed, int normal, float* out, void* stream);
+
+/* ---- Hartree CG ---------------------------------------------------------------
+ * replaces: src/Eigensolvers/pcg.py:7-85 (CPU) / pcg_gpu.py:22-65 (CuPy).
+ * Solves (-lap/h^2) x = rhs, FP64, warm start x, stop when r.r <= tol^2 r0.r0
+ * or maxit; alias_first=1 reproduces the CPU reference's first-iteration p/r
+ * aliasing.  Synchronises every check_every iterations. */
+int pc_pcg_workspace(void* grid, size_t* out_bytes);
+int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
+                 int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream);
+
+/* ---- ionic setup ------------------------------------------------------------------
+ * replaces: the per-point spline loops of src/V_ion/pseudoDiag.py:199-278
+ * (fsplevalIO, src/Splines/fsplevalIO.py:2-65). */
+int pc_sad_eval(void* grid, double h, double rad, int natoms, const double* pos, const int* species,
+                const int* tab_off, const int* tab_len, const double* xi, const double* z, const double* c,
+                const double* d, double* pot, double* rho, double* hpot0, void* stream);
+int pc_spline_eval(const double* xi, const double* z, const double* c, const double* d, int n, const double* r,
+                   long long m, double* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARSEC_HIP_H */

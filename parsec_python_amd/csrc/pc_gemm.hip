@@ -1,0 +1,324 @@
+// Rayleigh-Ritz block GEMMs on MFMA (gfx950).
+//
+//   gram  : S = X^T Y        X: n x kx, Y: n x ky (vector-major, ld), S FP64 kx x ky
+//   rotate: Z = X Q          X: n x kx, Q: FP64 kx x ky (column-major), Z: n x ky
+//
+// The reference forms G = W^T (H W) with a Python double loop of vdot
+// (src/Eigensolvers/chefsi1.py:51-53) or cuBLAS (chefsi1_gpu.py:61) and
+// rotates with W @ Q (chefsi1.py:67, chefsi1_gpu.py:66).
+//
+// MFMA 16x16x4 (f64: v_mfma_f64_16x16x4f64, f32: v_mfma_f32_16x16x4f32) lane
+// layout: A[m][k] from lane m + 16k, B[k][n] from lane n + 16k, D[m][n] in
+// lane n + 16*(m/4), register m%4.  The K index of an operand is only a label
+// that A and B must share, so each lane loads 4 consecutive points (one
+// 16-byte/32-byte load) and step s of 4 uses point 4*(lane/16)+s.
+//
+// gram splits the n points into chunks; each workgroup accumulates one output
+// tile over one chunk in FP32 (f32 inputs) or FP64 MFMA and writes an FP64
+// partial; a second kernel sums the partials in a fixed order (deterministic).
+#include "pc_common.h"
+#include <algorithm>
+
+namespace {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Acc;
+template <> struct Acc<double> {
+  typedef v4d type;
+  static __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Acc<float> {
+  typedef v4f type;
+  static __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// Load 4 consecutive points [p, p+4) of column col; zero beyond n.
+template <typename T, bool VEC>
+__device__ __forceinline__ void load4(const T* __restrict__ base, long long ld, int col, int ncol, long long p,
+                                      long long n, T out[4]) {
+  if (col < ncol) {
+    const T* src = base + (long long)col * ld + p;
+    if (VEC && p + 4 <= n) {
+      if constexpr (sizeof(T) == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(src);
+        out[0] = t.x; out[1] = t.y; out[2] = t.z; out[3] = t.w;
+      } else {
+        const double2 t0 = *reinterpret_cast<const double2*>(src);
+        const double2 t1 = *reinterpret_cast<const double2*>(src + 2);
+        out[0] = t0.x; out[1] = t0.y; out[2] = t1.x; out[3] = t1.y;
+      }
+      return;
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) out[s] = (p + s < n) ? src[s] : T(0);
+    return;
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) out[s] = T(0);
+}
+
+// ---------------------------------------------------------------- gram
+// Tile t -> (ti, tj): row-major over ti_n x tj_n, or the upper triangle
+// (ti <= tj) row by row when sym.
+__device__ __forceinline__ void tile_coords(int t, int ti_n, int tj_n, int sym, int& ti, int& tj) {
+  if (!sym) {
+    ti = t / tj_n;
+    tj = t - ti * tj_n;
+    return;
+  }
+  int i = 0;
+  while (t >= ti_n - i) {
+    t -= ti_n - i;
+    ++i;
+  }
+  ti = i;
+  tj = i + t;
+}
+
+// Workgroup = 4 waves, output tile 64x64 (each wave 32x32 = 2x2 MFMA tiles).
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void gram_kernel(const T* __restrict__ X, long long ldx, const T* __restrict__ Y,
+                                                   long long ldy, long long n, int kx, int ky, int ti_n, int tj_n,
+                                                   int ntile, int sym, int nchunk, long long chunk,
+                                                   double* __restrict__ part) {
+  typedef typename Acc<T>::type AT;
+  const int b = blockIdx.x;
+  const int tile = b % ntile;
+  const int ch = b / ntile;
+  int ti, tj;
+  tile_coords(tile, ti_n, tj_n, sym, ti, tj);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  const int i0 = ti * 64 + wi * 32, j0 = tj * 64 + wj * 32;
+  const int lm = lane & 15, lk = lane >> 4;
+  AT acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = AT{0, 0, 0, 0};
+  const long long pbeg = (long long)ch * chunk;
+  const long long pend = min(n, pbeg + chunk);
+  for (long long p = pbeg; p < pend; p += 16) {
+    T a[2][4], bb[2][4];
+    const long long pl = p + 4 * lk;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      load4<T, VEC>(X, ldx, i0 + 16 * t + lm, kx, pl, pend, a[t]);
+      load4<T, VEC>(Y, ldy, j0 + 16 * t + lm, ky, pl, pend, bb[t]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s)
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = Acc<T>::mfma(a[ta][s], bb[tb][s], acc[ta][tb]);
+  }
+  // partial layout: part[(ch*ntile + tile)*4096 + jj*64 + ii], tile-local 64x64 col-major
+  double* out = part + ((long long)ch * ntile + tile) * 4096;
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ii = wi * 32 + 16 * ta + 4 * lk + r;
+        const int jj = wj * 32 + 16 * tb + lm;
+        out[jj * 64 + ii] = (double)acc[ta][tb][r];
+      }
+}
+
+__global__ void gram_reduce(const double* __restrict__ part, int ntile, int nchunk, int ti_n, int tj_n, int kx,
+                            int ky, double* __restrict__ S, long long lds, int sym) {
+  const int tile = blockIdx.x;
+  int ti, tj;
+  tile_coords(tile, ti_n, tj_n, sym, ti, tj);
+  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+    const int ii = e & 63, jj = e >> 6;
+    const int i = ti * 64 + ii, j = tj * 64 + jj;
+    if (i >= kx || j >= ky) continue;
+    double s = 0.0;
+    for (int c = 0; c < nchunk; ++c) s += part[((long long)c * ntile + tile) * 4096 + e];
+    S[(long long)j * lds + i] = s;
+    if (sym && ti != tj) S[(long long)i * lds + j] = s;
+  }
+}
+
+// ---------------------------------------------------------------- rotate
+// Z^T (ky x n) = Q^T (ky x kx) * X^T (kx x n).  Workgroup = 4 waves; wave tile
+// = 64 output columns x 32 points (4x2 MFMA tiles); workgroup = 64 cols x 128 pts.
+template <typename T>
+__global__ __launch_bounds__(256) void rotate_kernel(const T* __restrict__ X, long long ldx,
+                                                     const T* __restrict__ Q, int ldq, T* __restrict__ Z,
+                                                     long long ldz, long long n, int kx, int ky, int npt_tiles,
+                                                     int ncol_tiles, int nblocks, int nb8) {
+  typedef typename Acc<T>::type AT;
+  const int L = (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
+  if (L >= nblocks) return;
+  const int ct = L % ncol_tiles;
+  const int pt = L / ncol_tiles;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int lm = lane & 15, lk = lane >> 4;
+  const int c0 = ct * 64;
+  const long long p0 = (long long)pt * 128 + w * 32;
+  AT acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) acc[a][c] = AT{0, 0, 0, 0};
+  for (int k = 0; k < kx; k += 4) {
+    const int kk = k + lk;
+    T qa[4], xb[2];
+#pragma unroll
+    for (int ta = 0; ta < 4; ++ta) {
+      const int col = c0 + 16 * ta + lm;
+      qa[ta] = (kk < kx && col < ky) ? Q[(long long)kk * ldq + col] : T(0);
+    }
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb) {
+      const long long p = p0 + 16 * tb + lm;
+      xb[tb] = (kk < kx && p < n) ? X[(long long)kk * ldx + p] : T(0);
+    }
+#pragma unroll
+    for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+      for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = Acc<T>::mfma(qa[ta], xb[tb], acc[ta][tb]);
+  }
+#pragma unroll
+  for (int ta = 0; ta < 4; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int col = c0 + 16 * ta + 4 * lk + r;
+        const long long p = p0 + 16 * tb + lm;
+        if (col < ky && p < n) Z[(long long)col * ldz + p] = acc[ta][tb][r];
+      }
+}
+
+__global__ void convert_q_f32(const double* __restrict__ Q, int ldq, int kx, int ky, float* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kx * ky) return;
+  const int i = e % kx, j = e / kx;
+  out[(long long)i * ky + j] = (float)Q[(long long)j * ldq + i];  // row-major kx x ky -> ldq' = ky
+}
+__global__ void convert_q_f64(const double* __restrict__ Q, int ldq, int kx, int ky, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= kx * ky) return;
+  const int i = e % kx, j = e / kx;
+  out[(long long)i * ky + j] = Q[(long long)j * ldq + i];
+}
+
+struct GramPlan {
+  int ti_n, tj_n, ntile, nchunk;
+  long long chunk;
+};
+
+GramPlan plan_gram(long long n, int kx, int ky, int sym) {
+  GramPlan p;
+  p.ti_n = (kx + 63) / 64;
+  p.tj_n = (ky + 63) / 64;
+  p.ntile = sym ? p.ti_n * (p.ti_n + 1) / 2 : p.ti_n * p.tj_n;
+  long long target = std::max<long long>(1, 2048 / p.ntile);
+  long long maxch = std::max<long long>(1, (n + 1023) / 1024);
+  p.nchunk = (int)std::min(target, maxch);
+  long long chunk = (n + p.nchunk - 1) / p.nchunk;
+  chunk = (chunk + 15) / 16 * 16;
+  p.chunk = chunk;
+  p.nchunk = (int)((n + chunk - 1) / chunk);
+  if (p.nchunk < 1) p.nchunk = 1;
+  return p;
+}
+
+size_t gram_work_bytes(const GramPlan& p) { return (size_t)p.nchunk * p.ntile * 4096 * sizeof(double); }
+
+template <typename T>
+int gram_impl(const T* X, long long ldx, const T* Y, long long ldy, long long n, int kx, int ky, double* S,
+              long long lds, int sym, void* work, size_t work_bytes, void* stream) {
+  PC_CHECK_ARG(X && Y && S && work, "null pointer");
+  PC_CHECK_ARG(n >= 1 && kx >= 1 && ky >= 1, "empty operand");
+  PC_CHECK_ARG(ldx >= n && ldy >= n && lds >= kx, "leading dimension too small");
+  PC_CHECK_ARG(!sym || (kx == ky), "sym requires kx == ky");
+  GramPlan p = plan_gram(n, kx, ky, sym);
+  PC_CHECK_ARG(work_bytes >= gram_work_bytes(p), "workspace too small");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  double* part = static_cast<double*>(work);
+  const bool vec = (ldx % 4 == 0) && (ldy % 4 == 0) &&
+                   (reinterpret_cast<uintptr_t>(X) % (4 * sizeof(T)) == 0) &&
+                   (reinterpret_cast<uintptr_t>(Y) % (4 * sizeof(T)) == 0);
+  dim3 grid(p.ntile * p.nchunk);
+  if (vec)
+    hipLaunchKernelGGL((gram_kernel<T, true>), grid, dim3(256), 0, st, X, ldx, Y, ldy, n, kx, ky, p.ti_n, p.tj_n,
+                       p.ntile, sym, p.nchunk, p.chunk, part);
+  else
+    hipLaunchKernelGGL((gram_kernel<T, false>), grid, dim3(256), 0, st, X, ldx, Y, ldy, n, kx, ky, p.ti_n, p.tj_n,
+                       p.ntile, sym, p.nchunk, p.chunk, part);
+  PC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(gram_reduce, dim3(p.ntile), dim3(256), 0, st, part, p.ntile, p.nchunk, p.ti_n, p.tj_n, kx, ky, S,
+                     lds, sym);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+template <typename T>
+int rotate_impl(const T* X, long long ldx, const double* Q, int ldq, T* Z, long long ldz, long long n, int kx,
+                int ky, void* work, size_t work_bytes, void* stream) {
+  PC_CHECK_ARG(X && Q && Z && work, "null pointer");
+  PC_CHECK_ARG(X != Z, "output must not alias input");
+  PC_CHECK_ARG(n >= 1 && kx >= 1 && ky >= 1, "empty operand");
+  PC_CHECK_ARG(ldx >= n && ldz >= n && ldq >= kx, "leading dimension too small");
+  PC_CHECK_ARG(work_bytes >= (size_t)kx * ky * sizeof(T), "workspace too small");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  T* qt = static_cast<T*>(work);
+  const int ne = kx * ky;
+  if constexpr (sizeof(T) == 4)
+    hipLaunchKernelGGL(convert_q_f32, dim3((ne + 255) / 256), dim3(256), 0, st, Q, ldq, kx, ky, qt);
+  else
+    hipLaunchKernelGGL(convert_q_f64, dim3((ne + 255) / 256), dim3(256), 0, st, Q, ldq, kx, ky, qt);
+  PC_LAUNCH_CHECK();
+  const int npt = (int)((n + 127) / 128);
+  const int nct = (ky + 63) / 64;
+  const int nb = npt * nct;
+  const int nb8 = (nb + 7) / 8;
+  hipLaunchKernelGGL((rotate_kernel<T>), dim3(8 * nb8), dim3(256), 0, st, X, ldx, qt, ky, Z, ldz, n, kx, ky, npt,
+                     nct, nb, nb8);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pc_gram_workspace(long long n, int kx, int ky, int sym, size_t* out_bytes) {
+  PC_CHECK_ARG(out_bytes, "null pointer");
+  *out_bytes = gram_work_bytes(plan_gram(n, kx, ky, sym));
+  return pc::PC_OK;
+}
+
+int pc_gram_f64(const double* X, long long ldx, const double* Y, long long ldy, long long n, int kx, int ky,
+                double* S, long long lds, int sym, void* work, size_t work_bytes, void* stream) {
+  return gram_impl<double>(X, ldx, Y, ldy, n, kx, ky, S, lds, sym, work, work_bytes, stream);
+}
+
+int pc_gram_f32(const float* X, long long ldx, const float* Y, long long ldy, long long n, int kx, int ky,
+                double* S, long long lds, int sym, void* work, size_t work_bytes, void* stream) {
+  return gram_impl<float>(X, ldx, Y, ldy, n, kx, ky, S, lds, sym, work, work_bytes, stream);
+}
+
+int pc_rotate_f64(const double* X, long long ldx, const double* Q, int ldq, double* Z, long long ldz, long long n,
+                  int kx, int ky, void* work, size_t work_bytes, void* stream) {
+  return rotate_impl<double>(X, ldx, Q, ldq, Z, ldz, n, kx, ky, work, work_bytes, stream);
+}
+
+int pc_rotate_f32(const float* X, long long ldx, const double* Q, int ldq, float* Z, long long ldz, long long n,
+                  int kx, int ky, void* work, size_t work_bytes, void* stream) {
+  return rotate_impl<float>(X, ldx, Q, ldq, Z, ldz, n, kx, ky, work, work_bytes, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,464 @@
+// Grid index map and the fused multi-vector finite-difference stencil.
+//
+// Storage ("compact, vector-major"): the domain is a set of x-rows (j,k) of an
+// nx*ny*nz cube, each row holding the contiguous points lo[j,k] <= i < hi[j,k]
+// (cube: the whole row; sphere: the chord).  Points are numbered k-major, then
+// j, then i, so row (j,k) starts at off[j,k].  Vector v of a block lives at
+// [v*ld, v*ld + npts).  Outside the domain values are zero (Dirichlet), which
+// is the reference's truncated-diagonal Laplacian (src/Laplacian/fd3d.py:79-100)
+// when the domain is the full cube.
+//
+// One launch computes, for every vector v and domain point p,
+//     out = sum_d w_d * (6 neighbours at distance d) + w_0 * y  (the -lap weights,
+//           pre-multiplied by the caller's scale)
+//         + (a*V[p] + b) * y[p] + c * x[p]
+// which is H*y (a=1,b=0,c=0) and every step of the reference's scaled and slim
+// Chebyshev filters (src/Eigensolvers/ch_filter.py:66-80, first_filt.py:94-101).
+// x may alias out (each point reads x before writing out at the same place).
+//
+// Kernel shape (CDNA4): a workgroup owns one vector, a slab of TY y-rows and
+// all x, and marches along z.  Each thread keeps its points' 2R+1 z-values in
+// registers; the current plane of the slab plus R halo rows above/below sits
+// in LDS for the x/y differences.  Workgroup ids are remapped so the slabs of
+// one vector run on the same XCD (its L2 then serves the halo rows).
+#include "pc_common.h"
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+struct GridDev {
+  int nx, ny, nz;
+  long long npts;
+  int* lo;       // [ny*nz]
+  int* hi;       // [ny*nz]
+  int* off;      // [ny*nz]
+  int* zrange;   // [2*ny]: min z, max z+1 of non-empty rows per y
+  int radius;
+  double w[8];   // 1D -d2/dx2 weights, centre first (unscaled)
+};
+
+struct StArgs {
+  int nx, ny, nz;
+  const int* lo;
+  const int* hi;
+  const int* off;
+  const int* zrange;
+  long long ld;
+  int nslab;
+  int nvec;
+  int nblocks;   // real (unpadded) number of workgroups
+  int nb8;       // ceil(nblocks/8)
+  double w[8];   // scaled weights; w[0] already includes the factor 3
+};
+
+template <typename T>
+__device__ __forceinline__ T row_val(const T* __restrict__ y, int lo, int hi, int off, int i) {
+  return (i >= lo && i < hi) ? y[off + (i - lo)] : T(0);
+}
+
+template <typename T, int R, int SX, int TYW>
+__global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
+                                                      T* Out, const T* __restrict__ V, T ca, T cb, T cc,
+                                                      double* __restrict__ dot_part) {
+  constexpr int NW = 4;
+  constexpr int TY = NW * TYW;
+  constexpr int NXP = 64 * SX;
+  constexpr int LW = NXP + 2 * R;   // LDS row stride (R zero columns each side)
+  constexpr int LR = TY + 2 * R;    // LDS rows
+  __shared__ T tile[LR * LW];
+  __shared__ double red[NW];
+
+  const int bid = blockIdx.x;
+  const int L = (bid & 7) * g.nb8 + (bid >> 3);
+  if (L >= g.nblocks) return;
+  const int v = L / g.nslab;
+  const int slab = L - v * g.nslab;
+  const int j0 = slab * TY;
+  const int lane = threadIdx.x & 63;
+  const int wv = threadIdx.x >> 6;
+
+  const T* __restrict__ yv = Y + (long long)v * g.ld;
+  const T* xv = X ? X + (long long)v * g.ld : nullptr;
+  T* ov = Out + (long long)v * g.ld;
+
+  // z-range of this slab's output rows.
+  int zb = g.nz, ze = 0;
+  for (int t = 0; t < TY; ++t) {
+    const int j = j0 + t;
+    if (j < g.ny) {
+      zb = min(zb, g.zrange[2 * j]);
+      ze = max(ze, g.zrange[2 * j + 1]);
+    }
+  }
+  double dot = 0.0;
+  if (zb >= ze) {
+    if (dot_part) {
+      double s = pc_block_sum(0.0, red);
+      if (threadIdx.x == 0) dot_part[L] = s;
+    }
+    return;
+  }
+
+  // zero the LDS (pad columns stay zero for the whole launch)
+  for (int t = threadIdx.x; t < LR * LW; t += 256) tile[t] = T(0);
+  __syncthreads();
+
+  T q[TYW][SX][2 * R + 1];
+  // prefill planes zb-R .. zb+R-1 into q[..][..][1..2R]
+#pragma unroll
+  for (int r = 0; r < TYW; ++r) {
+    const int j = j0 + wv + NW * r;
+#pragma unroll
+    for (int s = 0; s < SX; ++s) q[r][s][0] = T(0);
+#pragma unroll
+    for (int dz = 1; dz <= 2 * R; ++dz) {
+      const int z = zb - R - 1 + dz;
+      int lo = 0, hi = 0, off = 0;
+      if (j < g.ny && z >= 0 && z < g.nz) {
+        const int row = j + g.ny * z;
+        lo = g.lo[row]; hi = g.hi[row]; off = g.off[row];
+      }
+#pragma unroll
+      for (int s = 0; s < SX; ++s) q[r][s][dz] = row_val(yv, lo, hi, off, lane + 64 * s);
+    }
+  }
+
+  for (int z = zb; z < ze; ++z) {
+    // (1) shift the z-queues and load plane z+R
+#pragma unroll
+    for (int r = 0; r < TYW; ++r) {
+      const int j = j0 + wv + NW * r;
+      const int zz = z + R;
+      int lo = 0, hi = 0, off = 0;
+      if (j < g.ny && zz < g.nz) {
+        const int row = j + g.ny * zz;
+        lo = g.lo[row]; hi = g.hi[row]; off = g.off[row];
+      }
+#pragma unroll
+      for (int s = 0; s < SX; ++s) {
+#pragma unroll
+        for (int d = 0; d < 2 * R; ++d) q[r][s][d] = q[r][s][d + 1];
+        q[r][s][2 * R] = row_val(yv, lo, hi, off, lane + 64 * s);
+      }
+    }
+    // (2) plane z into LDS: interior rows from the queues, halo rows from memory
+#pragma unroll
+    for (int r = 0; r < TYW; ++r) {
+      const int tr = wv + NW * r + R;
+#pragma unroll
+      for (int s = 0; s < SX; ++s) tile[tr * LW + R + lane + 64 * s] = q[r][s][R];
+    }
+    for (int hr = wv; hr < 2 * R; hr += NW) {
+      const int tr = hr < R ? hr : hr + TY;
+      const int j = j0 - R + tr;
+      int lo = 0, hi = 0, off = 0;
+      if (j >= 0 && j < g.ny) {
+        const int row = j + g.ny * z;
+        lo = g.lo[row]; hi = g.hi[row]; off = g.off[row];
+      }
+#pragma unroll
+      for (int s = 0; s < SX; ++s) tile[tr * LW + R + lane + 64 * s] = row_val(yv, lo, hi, off, lane + 64 * s);
+    }
+    __syncthreads();
+    // (3) outputs
+#pragma unroll
+    for (int r = 0; r < TYW; ++r) {
+      const int j = j0 + wv + NW * r;
+      if (j < g.ny) {
+        const int row = j + g.ny * z;
+        const int lo = g.lo[row], hi = g.hi[row], off = g.off[row];
+        const int tr = wv + NW * r + R;
+#pragma unroll
+        for (int s = 0; s < SX; ++s) {
+          const int i = lane + 64 * s;
+          if (i >= lo && i < hi) {
+            const T yc = q[r][s][R];
+            T acc = T(g.w[0]) * yc;
+            const T* trow = tile + tr * LW + R + i;
+#pragma unroll
+            for (int d = 1; d <= R; ++d) {
+              const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[r][s][R - d] + q[r][s][R + d];
+              acc += T(g.w[d]) * nb;
+            }
+            const long long p = off + (i - lo);
+            T o = acc + (ca * V[p] + cb) * yc;
+            if (xv) o += cc * xv[p];
+            ov[p] = o;
+            if (dot_part) dot += (double)o * (double)yc;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (dot_part) {
+    double s = pc_block_sum(dot, red);
+    if (threadIdx.x == 0) dot_part[L] = s;
+  }
+}
+
+template <typename T, int R, int SX>
+int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+                double ca, double cb, double cc, double* dot_part, hipStream_t st) {
+  constexpr int TYW = (sizeof(T) == 4) ? (SX == 1 ? 8 : (SX == 2 ? 4 : 2)) : (SX == 1 ? 4 : (SX == 2 ? 2 : 1));
+  constexpr int TY = 4 * TYW;
+  StArgs a;
+  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
+  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange;
+  a.ld = ld;
+  a.nslab = (gd->ny + TY - 1) / TY;
+  a.nvec = nvec;
+  a.nblocks = a.nslab * nvec;
+  a.nb8 = (a.nblocks + 7) / 8;
+  for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
+  a.w[0] = 3.0 * gd->w[0] * scale;
+  for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
+  hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW>), dim3(8 * a.nb8), dim3(256), 0, st, a, Y, X, Out, V, T(ca), T(cb),
+                     T(cc), dot_part);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+template <typename T, int R>
+int launch_r(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+             double ca, double cb, double cc, double* dot_part, hipStream_t st) {
+  if (gd->nx <= 64) return launch_r_sx<T, R, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  if (gd->nx <= 128) return launch_r_sx<T, R, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  return launch_r_sx<T, R, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+}
+
+template <typename T>
+int stencil_dispatch(void* grid, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+                     double ca, double cb, double cc, double* dot_part, void* stream) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  PC_CHECK_ARG(gd != nullptr, "null grid");
+  PC_CHECK_ARG(Y && Out && V, "null operand");
+  PC_CHECK_ARG(Y != Out, "output must not alias the stencil input");
+  PC_CHECK_ARG(nvec >= 1, "nvec < 1");
+  PC_CHECK_ARG(ld >= gd->npts, "ld < npts");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (gd->radius) {
+    case 1: return launch_r<T, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 2: return launch_r<T, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 3: return launch_r<T, 3>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 4: return launch_r<T, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 6: return launch_r<T, 6>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    default:
+      pc::set_error("stencil: unsupported radius");
+      return pc::PC_ERR_ARG;
+  }
+}
+
+// Direct form for few vectors (Lanczos, CG): one wave per x-row, neighbours
+// read through the caches.  Used where the z-march has too few workgroups.
+template <typename T, int R>
+__global__ __launch_bounds__(256) void stencil_direct(StArgs g, const T* __restrict__ Y, const T* X, T* Out,
+                                                      const T* __restrict__ V, T ca, T cb, T cc,
+                                                      double* __restrict__ dot_part) {
+  __shared__ double red[4];
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int v = blockIdx.y;
+  const int nrow = g.ny * g.nz;
+  const T* __restrict__ yv = Y + (long long)v * g.ld;
+  const T* xv = X ? X + (long long)v * g.ld : nullptr;
+  T* ov = Out + (long long)v * g.ld;
+  double dot = 0.0;
+  if (row < nrow) {
+    const int j = row % g.ny, k = row / g.ny;
+    const int lo = g.lo[row], hi = g.hi[row], off = g.off[row];
+    for (int i = lo + lane; i < hi; i += 64) {
+      const long long p = off + (i - lo);
+      const T yc = yv[p];
+      T acc = T(g.w[0]) * yc;
+#pragma unroll
+      for (int d = 1; d <= R; ++d) {
+        T nb = row_val(yv, lo, hi, off, i - d) + row_val(yv, lo, hi, off, i + d);
+        if (j - d >= 0) {
+          const int r2 = row - d;
+          nb += row_val(yv, g.lo[r2], g.hi[r2], g.off[r2], i);
+        }
+        if (j + d < g.ny) {
+          const int r2 = row + d;
+          nb += row_val(yv, g.lo[r2], g.hi[r2], g.off[r2], i);
+        }
+        if (k - d >= 0) {
+          const int r2 = row - d * g.ny;
+          nb += row_val(yv, g.lo[r2], g.hi[r2], g.off[r2], i);
+        }
+        if (k + d < g.nz) {
+          const int r2 = row + d * g.ny;
+          nb += row_val(yv, g.lo[r2], g.hi[r2], g.off[r2], i);
+        }
+        acc += T(g.w[d]) * nb;
+      }
+      T o = acc + (ca * V[p] + cb) * yc;
+      if (xv) o += cc * xv[p];
+      ov[p] = o;
+      dot += (double)o * (double)yc;
+    }
+  }
+  if (dot_part) {
+    const double s = pc_block_sum(dot, red);
+    if (threadIdx.x == 0) dot_part[(long long)v * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+template <typename T, int R>
+int launch_direct_r(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld,
+                    double scale, double ca, double cb, double cc, double* dot_part, hipStream_t st) {
+  StArgs a;
+  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
+  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange;
+  a.ld = ld; a.nslab = 0; a.nvec = nvec; a.nblocks = 0; a.nb8 = 0;
+  for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
+  a.w[0] = 3.0 * gd->w[0] * scale;
+  for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
+  const int nrow = gd->ny * gd->nz;
+  dim3 grid((nrow + 3) / 4, nvec);
+  hipLaunchKernelGGL((stencil_direct<T, R>), grid, dim3(256), 0, st, a, Y, X, Out, V, T(ca), T(cb), T(cc), dot_part);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+template <typename T>
+int direct_dispatch(void* grid, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+                    double ca, double cb, double cc, double* dot_part, void* stream) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  PC_CHECK_ARG(gd != nullptr, "null grid");
+  PC_CHECK_ARG(Y && Out && V, "null operand");
+  PC_CHECK_ARG(Y != Out, "output must not alias the stencil input");
+  PC_CHECK_ARG(nvec >= 1 && nvec <= 65535, "nvec out of range");
+  PC_CHECK_ARG(ld >= gd->npts, "ld < npts");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  switch (gd->radius) {
+    case 1: return launch_direct_r<T, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 2: return launch_direct_r<T, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 3: return launch_direct_r<T, 3>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 4: return launch_direct_r<T, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 6: return launch_direct_r<T, 6>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    default:
+      pc::set_error("stencil: unsupported radius");
+      return pc::PC_ERR_ARG;
+  }
+}
+
+}  // namespace
+
+// Used by the CG solver in pc_pcg.hip.
+int pc_internal_stencil_direct_f64(void* grid, const double* Y, const double* X, double* Out, const double* V,
+                                   double scale, double ca, double cb, double cc, double* dot_part, void* stream,
+                                   int* out_nblocks) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  if (out_nblocks) *out_nblocks = (gd->ny * gd->nz + 3) / 4;
+  return direct_dispatch<double>(grid, Y, X, Out, V, 1, gd->npts, scale, ca, cb, cc, dot_part, stream);
+}
+
+long long pc_internal_grid_npts(void* grid) { return static_cast<const GridDev*>(grid)->npts; }
+int pc_internal_grid_view(void* grid, int* nx, int* ny, int* nz, const int** lo, const int** hi, const int** off) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  *nx = gd->nx; *ny = gd->ny; *nz = gd->nz;
+  *lo = gd->lo; *hi = gd->hi; *off = gd->off;
+  return 0;
+}
+long long pc_internal_grid_nrow(void* grid) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  return (long long)gd->ny * gd->nz;
+}
+
+extern "C" {
+
+int pc_stencil_direct_f64(void* grid, const double* Y, const double* X, double* Out, const double* V, int nvec,
+                          long long ld, double scale, double a, double b, double c, double* dot_part, void* stream) {
+  return direct_dispatch<double>(grid, Y, X, Out, V, nvec, ld, scale, a, b, c, dot_part, stream);
+}
+
+int pc_stencil_direct_f32(void* grid, const float* Y, const float* X, float* Out, const float* V, int nvec,
+                          long long ld, double scale, double a, double b, double c, double* dot_part, void* stream) {
+  return direct_dispatch<float>(grid, Y, X, Out, V, nvec, ld, scale, a, b, c, dot_part, stream);
+}
+
+int pc_grid_create(int nx, int ny, int nz, const int* row_lo, const int* row_hi, int radius, const double* weights,
+                   void** out_grid, long long* out_npts) {
+  PC_CHECK_ARG(out_grid && row_lo && row_hi && weights, "null pointer");
+  PC_CHECK_ARG(nx >= 1 && ny >= 1 && nz >= 1, "bad dims");
+  PC_CHECK_ARG(nx <= 256, "nx > 256 not supported (x rows are held by one workgroup)");
+  PC_CHECK_ARG(radius == 1 || radius == 2 || radius == 3 || radius == 4 || radius == 6,
+               "radius must be 1,2,3,4 or 6");
+  const int nrow = ny * nz;
+  std::vector<int> off(nrow), zr(2 * ny);
+  for (int j = 0; j < ny; ++j) { zr[2 * j] = nz; zr[2 * j + 1] = 0; }
+  long long cnt = 0;
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      const int r = j + ny * k;
+      PC_CHECK_ARG(row_lo[r] >= 0 && row_hi[r] <= nx, "row extent outside [0,nx]");
+      off[r] = (int)cnt;
+      const int len = std::max(0, row_hi[r] - row_lo[r]);
+      if (len > 0) {
+        zr[2 * j] = std::min(zr[2 * j], k);
+        zr[2 * j + 1] = std::max(zr[2 * j + 1], k + 1);
+      }
+      cnt += len;
+    }
+  PC_CHECK_ARG(cnt < (1LL << 31), "too many points");
+  // normalise empty rows to lo=hi=0
+  std::vector<int> lo(row_lo, row_lo + nrow), hi(row_hi, row_hi + nrow);
+  for (int r = 0; r < nrow; ++r)
+    if (hi[r] <= lo[r]) { lo[r] = 0; hi[r] = 0; }
+  GridDev* g = new GridDev();
+  g->nx = nx; g->ny = ny; g->nz = nz; g->npts = cnt; g->radius = radius;
+  for (int d = 0; d < 8; ++d) g->w[d] = d <= radius ? weights[d] : 0.0;
+  hipError_t e = hipSuccess;
+  e = hipMalloc(&g->lo, nrow * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&g->hi, nrow * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&g->off, nrow * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&g->zrange, 2 * ny * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(g->lo, lo.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->hi, hi.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->off, off.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(g->zrange, zr.data(), 2 * ny * sizeof(int), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    pc::set_error(std::string("pc_grid_create: ") + hipGetErrorString(e));
+    return pc::PC_ERR_HIP;
+  }
+  *out_grid = g;
+  if (out_npts) *out_npts = cnt;
+  return pc::PC_OK;
+}
+
+int pc_grid_destroy(void* grid) {
+  GridDev* g = static_cast<GridDev*>(grid);
+  if (!g) return pc::PC_OK;
+  (void)hipFree(g->lo); (void)hipFree(g->hi); (void)hipFree(g->off); (void)hipFree(g->zrange);
+  delete g;
+  return pc::PC_OK;
+}
+
+int pc_stencil_f64(void* grid, const double* Y, const double* X, double* Out, const double* V, int nvec,
+                   long long ld, double scale, double a, double b, double c, double* dot_part, void* stream) {
+  return stencil_dispatch<double>(grid, Y, X, Out, V, nvec, ld, scale, a, b, c, dot_part, stream);
+}
+
+int pc_stencil_f32(void* grid, const float* Y, const float* X, float* Out, const float* V, int nvec,
+                   long long ld, double scale, double a, double b, double c, double* dot_part, void* stream) {
+  return stencil_dispatch<float>(grid, Y, X, Out, V, nvec, ld, scale, a, b, c, dot_part, stream);
+}
+
+// Number of per-workgroup partial sums pc_stencil_* writes when dot_part is
+// given (padded layout: entry L of the logical workgroup order, vector-major).
+int pc_stencil_num_partials(void* grid, int dtype_bytes, int nvec, int* out_count, int* out_per_vector) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  PC_CHECK_ARG(gd && out_count && out_per_vector, "null pointer");
+  const int sx = gd->nx <= 64 ? 1 : (gd->nx <= 128 ? 2 : 4);
+  int tyw;
+  if (dtype_bytes == 4) tyw = sx == 1 ? 8 : (sx == 2 ? 4 : 2);
+  else tyw = sx == 1 ? 4 : (sx == 2 ? 2 : 1);
+  const int nslab = (gd->ny + 4 * tyw - 1) / (4 * tyw);
+  *out_per_vector = nslab;
+  *out_count = nslab * nvec;
+  return pc::PC_OK;
+}
+
+}  // extern "C"

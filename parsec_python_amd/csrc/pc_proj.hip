@@ -1,0 +1,183 @@
+// Kleinman-Bylander p-channel projectors in factored form.
+//
+// The reference stores V_nl = sum_a sum_{m=x,y,z} u_am u_am^T / (Zvalue/h^3) as
+// dense outer products in a sparse matrix (src/V_ion/pseudoNL_original.py:
+// 187-206).  Here each atom keeps its support points and the three u values,
+// and V_nl*Y is applied as
+//   gather : C[v, 3a+m] = sum_{p in supp(a)} u_am(p) * Y[p, v]          (FP64)
+//   scatter: Out[p, v] += alpha * sum_{(a,m) covering p} scale_a u_am(p) C[v, 3a+m]
+// The scatter runs over unique support points (no atomics, deterministic).
+#include "pc_common.h"
+#include <algorithm>
+#include <numeric>
+#include <vector>
+
+namespace {
+
+struct ProjDev {
+  int natoms;
+  int nent;
+  int nupts;
+  int* atom_ptr;   // [natoms+1]
+  int* ent_pt;     // [nent]  point index of entry
+  double* ent_u;   // [3*nent]
+  int* ent_atom;   // [nent]
+  double* scale;   // [natoms]
+  int* upt;        // [nupts] unique point index
+  int* upt_ptr;    // [nupts+1] into upt_ent
+  int* upt_ent;    // [nent] entry ids grouped by point
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void proj_gather(ProjDev P, const T* __restrict__ Y, long long ld, int nvec,
+                                                   double* __restrict__ C) {
+  const int a = blockIdx.x;
+  const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (v >= nvec) return;
+  const T* yv = Y + (long long)v * ld;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  const int e0 = P.atom_ptr[a], e1 = P.atom_ptr[a + 1];
+  for (int e = e0 + lane; e < e1; e += 64) {
+    const double y = (double)yv[P.ent_pt[e]];
+    s0 += P.ent_u[3 * e] * y;
+    s1 += P.ent_u[3 * e + 1] * y;
+    s2 += P.ent_u[3 * e + 2] * y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_down(s0, o, 64);
+    s1 += __shfl_down(s1, o, 64);
+    s2 += __shfl_down(s2, o, 64);
+  }
+  if (lane == 0) {
+    double* cv = C + (long long)v * 3 * P.natoms + 3 * a;
+    cv[0] = s0; cv[1] = s1; cv[2] = s2;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void proj_scatter(ProjDev P, const double* __restrict__ C, T* Out, long long ld,
+                                                    double alpha) {
+  const int q = blockIdx.x * 256 + threadIdx.x;
+  const int v = blockIdx.y;
+  if (q >= P.nupts) return;
+  const double* cv = C + (long long)v * 3 * P.natoms;
+  double s = 0.0;
+  for (int t = P.upt_ptr[q]; t < P.upt_ptr[q + 1]; ++t) {
+    const int e = P.upt_ent[t];
+    const int a = P.ent_atom[e];
+    const double* c = cv + 3 * a;
+    s += P.scale[a] * (P.ent_u[3 * e] * c[0] + P.ent_u[3 * e + 1] * c[1] + P.ent_u[3 * e + 2] * c[2]);
+  }
+  T* o = Out + (long long)v * ld + P.upt[q];
+  *o = (T)((double)(*o) + alpha * s);
+}
+
+template <typename T>
+int gather_impl(void* proj, const T* Y, long long ld, int nvec, double* C, void* stream) {
+  const ProjDev* P = static_cast<const ProjDev*>(proj);
+  PC_CHECK_ARG(P && Y && C, "null pointer");
+  if (P->natoms == 0 || nvec == 0) return pc::PC_OK;
+  dim3 grid(P->natoms, (nvec + 3) / 4);
+  hipLaunchKernelGGL(proj_gather<T>, grid, dim3(256), 0, static_cast<hipStream_t>(stream), *P, Y, ld, nvec, C);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+template <typename T>
+int scatter_impl(void* proj, const double* C, T* Out, long long ld, int nvec, double alpha, void* stream) {
+  const ProjDev* P = static_cast<const ProjDev*>(proj);
+  PC_CHECK_ARG(P && C && Out, "null pointer");
+  if (P->nupts == 0 || nvec == 0) return pc::PC_OK;
+  PC_CHECK_ARG(nvec <= 65535, "nvec > 65535");
+  dim3 grid((P->nupts + 255) / 256, nvec);
+  hipLaunchKernelGGL(proj_scatter<T>, grid, dim3(256), 0, static_cast<hipStream_t>(stream), *P, C, Out, ld, alpha);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// atom_ptr[natoms+1] groups entries by atom; ent_pt[nent] are point indices in
+// [0, npts); ent_u[3*nent] the (u_x,u_y,u_z) values; scale[natoms] = h^3/Zvalue.
+int pc_proj_create(int natoms, const int* atom_ptr, const int* ent_pt, const double* ent_u, const double* scale,
+                   long long npts, void** out_proj) {
+  PC_CHECK_ARG(out_proj && atom_ptr && scale, "null pointer");
+  PC_CHECK_ARG(natoms >= 0, "natoms < 0");
+  const int nent = atom_ptr[natoms];
+  PC_CHECK_ARG(nent >= 0 && (nent == 0 || (ent_pt && ent_u)), "bad entries");
+  std::vector<int> ent_atom(nent);
+  for (int a = 0; a < natoms; ++a) {
+    PC_CHECK_ARG(atom_ptr[a] <= atom_ptr[a + 1], "atom_ptr not monotone");
+    for (int e = atom_ptr[a]; e < atom_ptr[a + 1]; ++e) ent_atom[e] = a;
+  }
+  for (int e = 0; e < nent; ++e) PC_CHECK_ARG(ent_pt[e] >= 0 && ent_pt[e] < npts, "entry point out of range");
+  std::vector<int> order(nent);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ent_pt[x] < ent_pt[y]; });
+  std::vector<int> upt, upt_ptr;
+  for (int t = 0; t < nent; ++t) {
+    if (t == 0 || ent_pt[order[t]] != ent_pt[order[t - 1]]) {
+      upt.push_back(ent_pt[order[t]]);
+      upt_ptr.push_back(t);
+    }
+  }
+  upt_ptr.push_back(nent);
+  ProjDev* P = new ProjDev();
+  P->natoms = natoms; P->nent = nent; P->nupts = (int)upt.size();
+  hipError_t e = hipSuccess;
+  auto alloc_copy = [&](void** dst, const void* src, size_t bytes) {
+    if (e != hipSuccess) return;
+    e = hipMalloc(dst, std::max<size_t>(bytes, 8));
+    if (e == hipSuccess && bytes) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  alloc_copy((void**)&P->atom_ptr, atom_ptr, (natoms + 1) * sizeof(int));
+  alloc_copy((void**)&P->ent_pt, ent_pt, nent * sizeof(int));
+  alloc_copy((void**)&P->ent_u, ent_u, 3 * (size_t)nent * sizeof(double));
+  alloc_copy((void**)&P->ent_atom, ent_atom.data(), nent * sizeof(int));
+  alloc_copy((void**)&P->scale, scale, natoms * sizeof(double));
+  alloc_copy((void**)&P->upt, upt.data(), upt.size() * sizeof(int));
+  alloc_copy((void**)&P->upt_ptr, upt_ptr.data(), upt_ptr.size() * sizeof(int));
+  alloc_copy((void**)&P->upt_ent, order.data(), nent * sizeof(int));
+  if (e != hipSuccess) {
+    pc::set_error(std::string("pc_proj_create: ") + hipGetErrorString(e));
+    return pc::PC_ERR_HIP;
+  }
+  *out_proj = P;
+  return pc::PC_OK;
+}
+
+int pc_proj_destroy(void* proj) {
+  ProjDev* P = static_cast<ProjDev*>(proj);
+  if (!P) return pc::PC_OK;
+  (void)hipFree(P->atom_ptr); (void)hipFree(P->ent_pt); (void)hipFree(P->ent_u); (void)hipFree(P->ent_atom);
+  (void)hipFree(P->scale); (void)hipFree(P->upt); (void)hipFree(P->upt_ptr); (void)hipFree(P->upt_ent);
+  delete P;
+  return pc::PC_OK;
+}
+
+int pc_proj_ncols(void* proj, int* out_ncols) {
+  const ProjDev* P = static_cast<const ProjDev*>(proj);
+  PC_CHECK_ARG(P && out_ncols, "null pointer");
+  *out_ncols = 3 * P->natoms;
+  return pc::PC_OK;
+}
+
+int pc_proj_gather_f64(void* proj, const double* Y, long long ld, int nvec, double* C, void* stream) {
+  return gather_impl<double>(proj, Y, ld, nvec, C, stream);
+}
+int pc_proj_gather_f32(void* proj, const float* Y, long long ld, int nvec, double* C, void* stream) {
+  return gather_impl<float>(proj, Y, ld, nvec, C, stream);
+}
+int pc_proj_scatter_f64(void* proj, const double* C, double* Out, long long ld, int nvec, double alpha,
+                        void* stream) {
+  return scatter_impl<double>(proj, C, Out, ld, nvec, alpha, stream);
+}
+int pc_proj_scatter_f32(void* proj, const double* C, float* Out, long long ld, int nvec, double alpha,
+                        void* stream) {
+  return scatter_impl<float>(proj, C, Out, ld, nvec, alpha, stream);
+}
+
+}  // extern "C"

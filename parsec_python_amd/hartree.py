@@ -1,0 +1,42 @@
+"""Hartree potential: CG solve of (-lap/h^2) V_H = 4 pi (rho - rho0)/h^3 on the device.
+
+Mirrors ``pcg(A, rhs, x0, m, tol)`` (``src/Eigensolvers/pcg.py:7-85``; called at
+``src/rsdft_solver.py:345`` with m=200, tol=1e-5, warm start from the previous
+V_H).  FP64 throughout.  ``reference_aliasing=True`` reproduces the CPU
+reference's first-iteration p/r aliasing (its second search direction is
+(1+beta) r_1); False gives textbook CG (the reference's CuPy twin).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib
+from .device import Scratch, ptr, stream
+from .grid import DeviceGrid
+
+
+class HartreeSolver:
+    def __init__(self, dgrid: DeviceGrid, h: float, reference_aliasing: bool = True, check_every: int = 16):
+        self.dgrid = dgrid
+        self.inv_h2 = 1.0 / (h * h)
+        self.alias = int(bool(reference_aliasing))
+        self.check_every = check_every
+        nb = C.c_size_t()
+        _lib.call("pc_pcg_workspace", dgrid.handle, C.byref(nb))
+        self.work_bytes = nb.value
+        self.scratch = Scratch()
+        self.last_its = 0
+        self.last_res = 0.0
+
+    def solve(self, rhs: torch.Tensor, x: torch.Tensor, maxit: int = 200, tol: float = 1e-5) -> int:
+        """x (warm start, overwritten with the solution); returns iterations."""
+        work = self.scratch.bytes("pcg", self.work_bytes)
+        its = C.c_int()
+        res = C.c_double()
+        _lib.call("pc_pcg_solve", self.dgrid.handle, self.inv_h2, ptr(rhs), ptr(x), maxit, tol, self.alias,
+                  self.check_every, ptr(work), self.work_bytes, C.byref(its), C.byref(res), stream())
+        self.last_its = its.value
+        self.last_res = res.value
+        return its.value

@@ -1,0 +1,219 @@
+"""The SCF driver on one GPU (mirrors ``run_rsdft_calculation``,
+``src/rsdft_solver.py:156-462``, diagmeth 3).
+
+Per iteration (``src/rsdft_solver.py:281-395``):
+  H <- 0.5 A + V_nl + diag(pot)                      (resident operator, new pot)
+  iteration 1: first_filt(nev+15, H, poldeg); later: chefsi1(W, lam, poldeg, nev, H)
+  occupations(lam[:nev], T, Nelec, 1e-6)            (host FP64)
+  rho = sum_i 2 f_i W_i^2                            (device)
+  V_H <- CG(A, 4 pi/h^3 (rho - rho0), warm start)    (device FP64)
+  v_xc, pot_new, |pot_new - pot|/|pot_new|, energy sums (one device pass)
+  pot <- mixer(pot, pot_new - pot)                   (device)
+Energies in Ry as ``compute_energy_components`` (:128-153).  The run stops on
+the reference's rule err <= tol, and optionally on |E_i - E_{i-1}| < etol
+Ha/atom.
+
+Precision: ``fp64`` runs everything in FP64.  ``mixed`` keeps the wave
+functions, the filter, H*X and the subspace GEMMs in FP32 (MFMA, FP32
+accumulation within a chunk, FP64 across chunks) and keeps the Cholesky /
+eigensolver, density, Hartree CG, mixer and energy sums in FP64.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib, ionic
+from .device import F32, F64, ptr, require_gpu, stream
+from .eigensolvers import Eigensolver
+from .grid import Grid, make_grid, reference_fd_radius
+from .hamiltonian import Hamiltonian, leading_dim
+from .hartree import HartreeSolver
+from .mixer import DeviceMsecant1
+from .occupations import occupations
+
+RY_EV = 13.605698066
+
+
+@dataclass
+class SCFConfig:
+    fd_order: int = 8
+    fd_radius: int | None = None       # explicit stencil radius (e.g. 6 for order 12)
+    maxits: int = 40
+    tol: float = 2e-4
+    fermi_temp: float = 500.0
+    poldeg: int = 10
+    precision: str = "fp64"            # "fp64" | "mixed"
+    domain: str = "cube"               # "cube" | "sphere"
+    seed: int = 0
+    etol_ha_per_atom: float | None = None
+    pcg_aliasing: bool = True
+    pcg_maxit: int = 200
+    pcg_tol: float = 1e-5
+    verbose: bool = False
+
+
+@dataclass
+class SCFResult:
+    energies: dict
+    eigenvalues: np.ndarray
+    occupations: np.ndarray
+    iterations: int
+    error: float
+    converged: bool
+    history: list
+    timings: dict
+    rho: torch.Tensor            # e/bohr^3 on the domain
+    hpot: torch.Tensor
+    pot: torch.Tensor
+    W: torch.Tensor              # (k, ld) block, rows = states
+    grid: Grid
+    setup_info: dict = field(default_factory=dict)
+
+
+class SCFProblem:
+    """Everything the SCF loop needs, built once (setup)."""
+
+    def __init__(self, atoms, domain: dict, nev: int, cfg: SCFConfig):
+        require_gpu()
+        t0 = time.perf_counter()
+        self.cfg = cfg
+        self.atoms = ionic.normalize_atoms(atoms)
+        self.nev = int(nev)
+        self.k = self.nev + 15
+        self.grid = make_grid(int(domain["nx"]), float(domain["h"]), float(domain["radius"]), cfg.domain)
+        self.h = self.grid.h
+        self.n = self.grid.npts
+        self.radius = cfg.fd_radius if cfg.fd_radius is not None else reference_fd_radius(cfg.fd_order)
+        self.dtype = F64 if cfg.precision == "fp64" else F32
+        self.proj = ionic.DeviceProjectors(self.grid, self.atoms)
+        pot0 = torch.zeros(self.n, dtype=F64, device="cuda")
+        self.H = Hamiltonian(self.grid, self.radius, self.proj, pot0, self.dtype)
+        self.rho0, self.hpot0, self.ppot, self.diag_info = ionic.pseudo_diag(self.grid, self.H.dgrid, self.atoms)
+        self.e_nuc = ionic.nuclear(self.atoms)
+        self.nelec = ionic.nelectrons(self.atoms)
+        self.n_atoms = sum(x.shape[0] for _, x in self.atoms)
+        torch.cuda.synchronize()
+        self.setup_time = time.perf_counter() - t0
+
+    def close(self):
+        self.H.close()
+        self.proj.close()
+
+
+def _xc(n, rho, vxc, part, out1):
+    _lib.call("pc_xc", n, ptr(rho), ptr(vxc), ptr(part), ptr(out1), stream())
+
+
+def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: torch.Tensor | None = None,
+            max_wall: float | None = None) -> SCFResult:
+    """Run the SCF loop.  ``rho_init``/``hpot_init`` replace (rho0, V_H0) for a
+    learned start, as the reference's ML density path does
+    (``src/V_ion/pseudoDiag_ML4Den_poisson.py:392-540``)."""
+    cfg = prob.cfg
+    n, h, nev, k = prob.n, prob.h, prob.nev, prob.k
+    h3 = h ** 3
+    H = prob.H
+    rho0 = prob.rho0 if rho_init is None else rho_init
+    hpot0 = prob.hpot0 if hpot_init is None else hpot_init
+    ppot = prob.ppot
+    dev = dict(dtype=F64, device="cuda")
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+
+    import ctypes as C
+    nbc = C.c_int()
+    _lib.call("pc_num_blocks", n, C.byref(nbc))
+    nb = nbc.value
+    part = torch.empty(5 * nb, **dev)
+    sums = torch.empty(8, **dev)
+    vxc = torch.empty(n, **dev)
+    rhoxc = rho0 / h3
+    _xc(n, rhoxc, vxc, part, sums)
+    pot = ppot + hpot0 + 0.5 * vxc
+    hpot = torch.zeros(n, **dev)
+    pot_new = torch.empty(n, **dev)
+    rho_cell = torch.empty(n, **dev)
+    rho = torch.empty(n, **dev)
+    rhs = torch.empty(n, **dev)
+    f1 = torch.empty(n, **dev)
+    pot_next = torch.empty(n, **dev)
+    occ2_dev = torch.empty(k, **dev)
+
+    eig = Eigensolver(H, k, seed=cfg.seed)
+    hart = HartreeSolver(H.dgrid, h, reference_aliasing=cfg.pcg_aliasing)
+    mixer = DeviceMsecant1(n, cfg.maxits + 2)
+
+    err = 10.0 + cfg.tol
+    its = 0
+    W = None
+    lam = None
+    occ = None
+    hist = []
+    e_prev = None
+    energies = None
+    converged = False
+    tim = {"diag": 0.0, "hartree": 0.0, "post": 0.0}
+    while err > cfg.tol and its < cfg.maxits:
+        its += 1
+        t0 = time.perf_counter()
+        H.set_potential(pot)
+        if its == 1:
+            W, lam = eig.first_filt(k, cfg.poldeg)
+        else:
+            W, lam = eig.chefsi1(W, lam, cfg.poldeg, nev)
+        t1 = time.perf_counter()
+        _, occ = occupations(lam[:nev], cfg.fermi_temp, prob.nelec, 1e-6)
+        occ2_dev[:nev].copy_(torch.from_numpy(2.0 * np.asarray(occ, dtype=np.float64)))
+        sfx = "f64" if W.dtype == F64 else "f32"
+        _lib.call(f"pc_density_{sfx}", ptr(W), W.shape[1], n, nev, ptr(occ2_dev), ptr(rho_cell), stream())
+        _lib.call("pc_hartree_rhs", n, ptr(rho_cell), ptr(rho0), 4 * math.pi / h3, ptr(rhs), stream())
+        cg_its = hart.solve(rhs, hpot, cfg.pcg_maxit, cfg.pcg_tol)
+        t2 = time.perf_counter()
+        _lib.call("pc_scf_post", n, ptr(rho_cell), h3, ptr(ppot), ptr(hpot), ptr(hpot0), ptr(pot), ptr(rho),
+                  ptr(vxc), ptr(pot_new), ptr(part), ptr(sums), stream())
+        s = sums[:5].cpu().numpy()
+        err = float(math.sqrt(s[1]) / math.sqrt(s[2]))
+        esum = float(np.sum(lam[:nev] * occ[:nev]))
+        e = {"eigen_sum_ry": 4.0 * esum, "hartree_ry": float(s[3] * h3), "xc_ry": float(s[0] * h3),
+             "ion_ry": float(prob.e_nuc)}
+        vxc_sum = float(s[4] * h3)
+        e["total_ry"] = e["eigen_sum_ry"] - e["hartree_ry"] + e["xc_ry"] - vxc_sum + e["ion_ry"]
+        energies = e
+        e_ha_atom = 0.5 * e["total_ry"] / prob.n_atoms
+        de = None if e_prev is None else abs(e_ha_atom - e_prev)
+        e_prev = e_ha_atom
+        hist.append({"iter": its, "err": err, "total_ry": e["total_ry"], "dE_ha_atom": de, "cg_its": cg_its,
+                     "lam": np.array(lam[:nev])})
+        # pot <- mixer(pot, pot_new - pot)
+        f1.copy_(pot_new)
+        _lib.call("pc_lincomb", n, None, n, 0, None, 1.0, ptr(pot), -1.0, ptr(f1), stream())
+        mixer(pot, f1, pot_next)
+        pot, pot_next = pot_next, pot
+        t3 = time.perf_counter()
+        tim["diag"] += t1 - t0
+        tim["hartree"] += t2 - t1
+        tim["post"] += t3 - t2
+        if cfg.verbose:
+            print(f"  SCF {its:3d}  err {err:9.2e}  E {e['total_ry']:.8f} Ry  dE {de if de is not None else 0:.2e}"
+                  f" Ha/atom  CG {cg_its}")
+        if cfg.etol_ha_per_atom is not None and de is not None and de < cfg.etol_ha_per_atom:
+            converged = True
+            break
+        if max_wall is not None and time.perf_counter() - t_start > max_wall:
+            break
+    if not converged:
+        converged = err <= cfg.tol
+    torch.cuda.synchronize()
+    tim["total"] = time.perf_counter() - t_start
+    tim["per_iteration"] = tim["total"] / max(its, 1)
+    res = SCFResult(energies=energies, eigenvalues=np.asarray(lam), occupations=np.asarray(occ), iterations=its,
+                    error=err, converged=converged, history=hist, timings=tim, rho=rho, hpot=hpot, pot=pot, W=W,
+                    grid=prob.grid, setup_info={"setup_time": prob.setup_time, **prob.diag_info,
+                                                "filter_steps": eig.stats["filter_steps"], "rr": eig.stats["rr"]})
+    eig.close()
+    return res

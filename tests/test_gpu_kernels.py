@@ -1,0 +1,261 @@
+"""GPU parity tests: each HIP entry point against the FP64 oracle (or a closed
+form) on seeded inputs.  Tolerances: FP64 paths 1e-11 relative (reduction order
+only), FP32 paths 2e-5 relative to the operand scale."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from oracle import fd as ofd
+from oracle import grid as ogrid
+from oracle import pseudo as opseudo
+from oracle import xc as oxc
+from oracle.pcg import pcg as opcg
+
+pytestmark = pytest.mark.gpu
+
+
+def _grid(nx, ny, nz, h, kind):
+    from parsec_python_amd.grid import Grid, make_grid
+    if kind == "cube" and not (nx == ny == nz):
+        return Grid(nx=nx, ny=ny, nz=nz, h=h, rad=0.5 * h * (nx - 1), mask=np.ones(nx * ny * nz, bool), kind="cube")
+    return make_grid(nx, h, 0.5 * h * (nx - 1), kind)
+
+
+def _oracle_lap(g, radius):
+    return ofd.laplacian_masked(g.nx, g.ny, g.nz, radius, g.mask)
+
+
+CASES = [  # nx, ny, nz, kind, radius
+    (20, 20, 20, "cube", 4), (21, 21, 21, "sphere", 4), (18, 18, 18, "sphere", 6), (16, 16, 16, "cube", 1),
+    (17, 17, 17, "cube", 2), (19, 19, 19, "sphere", 3), (70, 12, 9, "cube", 4), (130, 9, 7, "cube", 6),
+]
+
+
+@pytest.mark.parametrize("nx,ny,nz,kind,radius", CASES)
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("nvec", [2, 9])
+def test_stencil_step_vs_oracle(gpu, nx, ny, nz, kind, radius, dtype, nvec):
+    import torch
+    from parsec_python_amd.hamiltonian import Hamiltonian, leading_dim
+    print(f"stencil case {nx}x{ny}x{nz} {kind} r={radius} {dtype} nvec={nvec}", flush=True)
+    h = 0.45
+    g = _grid(nx, ny, nz, h, kind)
+    n = g.npts
+    rng = np.random.default_rng(nx * 31 + radius)
+    V = rng.standard_normal(n)
+    Y = rng.standard_normal((nvec, n))
+    X = rng.standard_normal((nvec, n))
+    a, b, c = 0.7, -0.3, -1.1
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    H = Hamiltonian(g, radius, None, torch.tensor(V, device="cuda"), tdt)
+    ld = leading_dim(n)
+    Yd = torch.zeros((nvec, ld), dtype=tdt, device="cuda")
+    Xd = torch.zeros((nvec, ld), dtype=tdt, device="cuda")
+    Yd[:, :n] = torch.tensor(Y, dtype=tdt)
+    Xd[:, :n] = torch.tensor(X, dtype=tdt)
+    # in-place form: out aliases X
+    H.step(Yd, Xd, Xd, nvec, a, b, c)
+    got = Xd[:, :n].double().cpu().numpy()
+    L = _oracle_lap(g, radius) * (0.5 / h ** 2)
+    Yr = Y.astype(np.float32).astype(np.float64) if dtype == "f32" else Y
+    Xr = X.astype(np.float32).astype(np.float64) if dtype == "f32" else X
+    Vr = V.astype(np.float32).astype(np.float64) if dtype == "f32" else V
+    ref = a * (L @ Yr.T).T + (a * Vr + b) * Yr + c * Xr
+    tol = 1e-11 if dtype == "f64" else 2e-5
+    scale = np.abs(ref).max()
+    assert np.max(np.abs(got - ref)) <= tol * scale, np.max(np.abs(got - ref)) / scale
+    H.close()
+
+
+def test_projectors_vs_oracle(gpu):
+    import torch
+    from parsec_python_amd import ionic, molecules as M
+    from parsec_python_amd.grid import make_grid
+    from parsec_python_amd.hamiltonian import Hamiltonian, leading_dim
+    atoms, dom, _ = M.prepare(M.h2o_sample(), h_ang=0.3, radius_ang=4.0)
+    for kind in ("cube", "sphere"):
+        g = make_grid(dom["nx"], dom["h"], dom["radius"], kind)
+        proj = ionic.DeviceProjectors(g, atoms)
+        tabs = opseudo.projector_tables(dom, atoms, g.mask)
+        assert proj.natoms == len(tabs)
+        for t, (lo, hi) in zip(tabs, zip(proj.tables[0][:-1], proj.tables[0][1:])):
+            assert np.array_equal(t["idx"], proj.tables[1][lo:hi])
+            assert np.allclose(t["u"], proj.tables[2][lo:hi], rtol=1e-13, atol=1e-16)
+        n = g.npts
+        nvec = 5
+        rng = np.random.default_rng(7)
+        Y = rng.standard_normal((nvec, n))
+        H = Hamiltonian(g, 4, proj, torch.zeros(n, dtype=torch.float64, device="cuda"), torch.float64)
+        ld = leading_dim(n)
+        Yd = torch.zeros((nvec, ld), dtype=torch.float64, device="cuda")
+        Yd[:, :n] = torch.tensor(Y)
+        out = torch.zeros_like(Yd)
+        H.apply(Yd, out, nvec)
+        L = ofd.laplacian_masked(g.nx, g.ny, g.nz, 4, g.mask) * (0.5 / g.h ** 2)
+        ref = (L @ Y.T).T + opseudo.apply_nl(tabs, Y.T).T
+        got = out[:, :n].cpu().numpy()
+        assert np.max(np.abs(got - ref)) <= 1e-11 * np.abs(ref).max()
+        H.close()
+        proj.close()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("n,k", [(1000, 23), (5003, 49), (20000, 70)])
+def test_gram_and_rotate(gpu, dtype, n, k):
+    import torch
+    from parsec_python_amd.hamiltonian import BlockLinalg, leading_dim
+    print(f"gram case n={n} k={k} {dtype}", flush=True)
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    rng = np.random.default_rng(n + k)
+    X = rng.standard_normal((k, n))
+    Y = rng.standard_normal((k + 3, n))
+    ld = leading_dim(n)
+    Xd = torch.zeros((k, ld), dtype=tdt, device="cuda")
+    Yd = torch.zeros((k + 3, ld), dtype=tdt, device="cuda")
+    Xd[:, :n] = torch.tensor(X, dtype=tdt)
+    Yd[:, :n] = torch.tensor(Y, dtype=tdt)
+    Xr = Xd[:, :n].double().cpu().numpy()
+    Yr = Yd[:, :n].double().cpu().numpy()
+    la = BlockLinalg(n, k + 3)
+    S = torch.empty((k, k), dtype=torch.float64, device="cuda")
+    la.gram(Xd, Xd, k, k, S.view(-1), sym=True)
+    G = torch.empty((k + 3, k), dtype=torch.float64, device="cuda")
+    la.gram(Xd, Yd, k, k + 3, G.view(-1), sym=False)
+    Sref = Xr @ Xr.T
+    Gref = Xr @ Yr.T  # [i, j] = X_i . Y_j ; stored column-major (j-major rows)
+    tol = 1e-12 if dtype == "f64" else 3e-6
+    assert np.max(np.abs(S.cpu().numpy() - Sref)) <= tol * np.abs(Sref).max() * (1 if dtype == "f64" else 10)
+    assert np.max(np.abs(G.cpu().numpy().T - Gref)) <= tol * np.abs(Sref).max() * (1 if dtype == "f64" else 10)
+    Q = rng.standard_normal((k + 3, k))  # column-major k x (k+3) -> row j = column j
+    Qd = torch.tensor(Q, device="cuda")
+    Z = torch.zeros((k + 3, ld), dtype=tdt, device="cuda")
+    la.rotate(Xd, Qd.view(-1), k, k + 3, Z)
+    Zref = (Xr.T @ Q.T).T
+    ztol = 1e-12 if dtype == "f64" else 2e-6
+    assert np.max(np.abs(Z[:, :n].double().cpu().numpy() - Zref)) <= ztol * np.abs(Zref).max()
+    la.close()
+
+
+def test_dense_rr_generalized_eig(gpu):
+    import torch
+    from scipy.linalg import eigh
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    rng = np.random.default_rng(5)
+    k = 40
+    B = rng.standard_normal((200, k))
+    S = B.T @ B
+    M = rng.standard_normal((k, k))
+    G = M + M.T
+    la = BlockLinalg(10, k)
+    Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
+    Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
+    T, lam = la.dense_rr(Sd, Gd, k)
+    ref = eigh(G, S, eigvals_only=True)
+    assert np.allclose(lam, ref, rtol=1e-10, atol=1e-10)
+    Tm = T.cpu().numpy().reshape(k, k).T  # column-major
+    assert np.allclose(Tm.T @ S @ Tm, np.eye(k), atol=1e-9)
+    assert np.allclose(Tm.T @ G @ Tm, np.diag(lam), atol=1e-8)
+    la.close()
+
+
+def test_density_xc_post_vs_oracle(gpu):
+    import torch
+    from parsec_python_amd import _lib
+    from parsec_python_amd.device import ptr, stream
+    rng = np.random.default_rng(11)
+    n, nev, ld = 4099, 6, 4112
+    W = rng.standard_normal((nev + 2, ld)) * 0.01
+    occ2 = rng.uniform(0, 2, nev)
+    Wd = torch.tensor(W, device="cuda")
+    o2 = torch.tensor(occ2, device="cuda")
+    rho_cell = torch.empty(n, dtype=torch.float64, device="cuda")
+    _lib.call("pc_density_f64", ptr(Wd), ld, n, nev, ptr(o2), ptr(rho_cell), stream())
+    ref = (W[:nev, :n] ** 2 * occ2[:, None]).sum(0)
+    assert np.allclose(rho_cell.cpu().numpy(), ref, rtol=1e-13, atol=1e-18)
+    # include zero / tiny / large densities (both PZ81 branches)
+    rc = np.abs(rng.standard_normal(n)) * 0.05
+    rc[:10] = 0.0
+    rc[10:20] = 1e-12
+    h3 = 0.3 ** 3
+    ppot, hpot, hpot0, pot = (rng.standard_normal(n) for _ in range(4))
+    dv = [torch.tensor(v, device="cuda") for v in (rc, ppot, hpot, hpot0, pot)]
+    rho = torch.empty(n, dtype=torch.float64, device="cuda")
+    vxc = torch.empty_like(rho)
+    pn = torch.empty_like(rho)
+    nb = C.c_int()
+    _lib.call("pc_num_blocks", n, C.byref(nb))
+    part = torch.empty(5 * nb.value, dtype=torch.float64, device="cuda")
+    sums = torch.empty(5, dtype=torch.float64, device="cuda")
+    _lib.call("pc_scf_post", n, ptr(dv[0]), h3, ptr(dv[1]), ptr(dv[2]), ptr(dv[3]), ptr(dv[4]), ptr(rho), ptr(vxc),
+              ptr(pn), ptr(part), ptr(sums), stream())
+    rr = rc / h3
+    v_ref, e_ref = oxc.exc_nspn(1.0, rr)
+    assert np.allclose(vxc.cpu().numpy(), v_ref, rtol=1e-13, atol=1e-15)
+    pn_ref = ppot + 0.5 * v_ref + hpot + hpot0
+    assert np.allclose(pn.cpu().numpy(), pn_ref, rtol=1e-14, atol=1e-14)
+    s = sums.cpu().numpy()
+    assert s[0] == pytest.approx(e_ref, rel=1e-12)
+    assert s[1] == pytest.approx(np.sum((pn_ref - pot) ** 2), rel=1e-12)
+    assert s[2] == pytest.approx(np.sum(pn_ref ** 2), rel=1e-12)
+    assert s[3] == pytest.approx(np.sum(rr * (hpot + hpot0)), rel=1e-11)
+    assert s[4] == pytest.approx(np.sum(rr * v_ref), rel=1e-12)
+
+
+@pytest.mark.parametrize("kind", ["cube", "sphere"])
+@pytest.mark.parametrize("alias", [True, False])
+def test_pcg_vs_oracle(gpu, kind, alias):
+    import torch
+    from parsec_python_amd.grid import DeviceGrid, make_grid
+    from parsec_python_amd.hartree import HartreeSolver
+    h = 0.4
+    g = make_grid(24, h, 0.5 * h * 23, kind)
+    n = g.npts
+    A = (1.0 / h ** 2) * ofd.laplacian_masked(g.nx, g.ny, g.nz, 4, g.mask)
+    rng = np.random.default_rng(3)
+    rhs = rng.standard_normal(n)
+    x0 = rng.standard_normal(n) * 0.1
+    xr, its_r = opcg(A, rhs, x0, 200, 1e-5, reference_aliasing=alias)
+    dg = DeviceGrid(g, 4)
+    hs = HartreeSolver(dg, h, reference_aliasing=alias, check_every=7)
+    x = torch.tensor(x0, device="cuda")
+    its = hs.solve(torch.tensor(rhs, device="cuda"), x, 200, 1e-5)
+    assert its == its_r
+    assert np.max(np.abs(x.cpu().numpy() - xr)) <= 1e-9 * np.abs(xr).max()
+    # zero iterations when the warm start is exact
+    x2 = torch.tensor(xr, device="cuda")
+    its2 = hs.solve(torch.tensor(A @ xr, device="cuda"), x2, 200, 1e-5)
+    assert its2 <= 1
+    dg.close()
+
+
+@pytest.mark.parametrize("kind", ["cube", "sphere"])
+def test_sad_eval_vs_oracle(gpu, kind):
+    from parsec_python_amd import ionic, molecules as M
+    from parsec_python_amd.grid import DeviceGrid, make_grid
+    atoms, dom, _ = M.prepare(M.h2o_sample(), h_ang=0.2, radius_ang=5.0)
+    g = make_grid(dom["nx"], dom["h"], dom["radius"], kind)
+    dg = DeviceGrid(g, 4)
+    rho0, hpot0, pot, info = ionic.pseudo_diag(g, dg, atoms)
+    r0, h0, p0, i0 = opseudo.pseudo_diag(dom, atoms, g.mask)
+    assert np.allclose(rho0.cpu().numpy(), r0, rtol=1e-12, atol=1e-16)
+    assert np.allclose(hpot0.cpu().numpy(), h0, rtol=1e-12, atol=1e-14)
+    assert np.allclose(pot.cpu().numpy(), p0, rtol=1e-12, atol=1e-14)
+    assert info["electron_count_initial"] == pytest.approx(i0["electron_count_initial"], rel=1e-13)
+    if kind == "cube":
+        gold = np.load("tests/golden/h2o_sample_rho_grid.npz")["init"]
+        assert np.max(np.abs(rho0.cpu().numpy() - gold)) < 1e-15
+    dg.close()
+
+
+def test_rand_matches_mirror(gpu):
+    import torch
+    from parsec_python_amd.eigensolvers import rand_block
+    from tests.rng_mirror import block
+    n, k = 1001, 3
+    out = torch.empty((k, 1008), dtype=torch.float64, device="cuda")
+    rand_block(out, n, k, 12345, normal=False)
+    assert np.array_equal(out[:, :n].cpu().numpy(), block(12345, n, k, False).T)
+    rand_block(out, n, k, 777, normal=True)
+    assert np.allclose(out[:, :n].cpu().numpy(), block(777, n, k, True).T, rtol=1e-14, atol=1e-14)

@@ -1,0 +1,298 @@
+"""CPU tests: pin the FP64 oracle to the reference's own run logs and fixtures
+and to closed forms / library routines (no GPU needed)."""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.interpolate import CubicSpline
+
+from oracle import eig, fd, grid, pseudo, scf, splines, tables, xc
+from oracle.mixer import Msecant1
+from oracle.pcg import pcg
+from parsec_python_amd import molecules as M
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+LOGS = json.load(open(os.path.join(GOLD, "reference_logs.json")))
+RY_EV = 13.605698066
+
+
+def _h2o_sample():
+    return M.prepare(M.h2o_sample(), h_ang=0.2, radius_ang=5.0)
+
+
+def _h_smoke():
+    return M.prepare(M.h_atom(), h_ang=None, radius_ang=None, nev=1)
+
+
+# ------------------------------------------------------------------ sizing (P12)
+def test_domain_sizing_matches_logs():
+    atoms, dom, nev = _h2o_sample()
+    assert dom["nx"] == LOGS["h2o_sample"]["nx"] and nev == LOGS["h2o_sample"]["nev"]
+    assert abs(dom["radius"] - LOGS["h2o_sample"]["radius_bohr"]) < 1e-5
+    atoms, dom, nev = _h_smoke()
+    assert dom["nx"] == 22 and abs(dom["radius"] - 6.3) < 1e-12 and abs(dom["h"] - 0.6) < 1e-15
+    atoms, dom, nev = M.prepare(M.naphthalene_sample(), h_ang=0.2, radius_ang=9.0)
+    assert dom["nx"] == 92 and nev == 34 and abs(dom["radius"] - 17.19651) < 1e-5
+    # oracle's own restatement agrees with the product's
+    o = grid.size_domain(atoms, dom["h"], 9.0 / grid.A0_ANG)
+    assert o["nx"] == 92 and o["nev"] == 34
+
+
+def test_sphere_point_counts():
+    for ang, h, n in [(5.0, 0.2, 69264), (9.0, 0.2, 394264), (13.0, 0.25, 605808)]:
+        _, dom, _ = M.prepare(M.h2o(), h_ang=h, radius_ang=ang)
+        assert grid.mask_flat(dom, "sphere").sum() == n
+    assert grid.index_sphere_mask(128, 63.5, 60.0).sum() == 904960
+
+
+# ------------------------------------------------------------------ FD weights (P3)
+@pytest.mark.parametrize("p", [1, 2, 3, 4])
+def test_fd_closed_form_equals_reference_fractions(p):
+    assert fd.fd_coefficients(p) == fd.REFERENCE_WEIGHTS[p]
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 6])
+def test_fd_taylor_order(p):
+    """sum_k c_k k^(2m) = 0 for 1 <= m < p, and the m=1 moment gives -d2/dx2."""
+    w = fd.fd_coefficients(p)
+    assert 2 * sum(w[k] * k ** 2 for k in range(1, p + 1)) == Fraction(-2)
+    for m in range(2, p + 1):
+        assert sum(w[k] * k ** (2 * m) for k in range(1, p + 1)) == 0
+    assert sum(w[k] * k ** (2 * p + 2) for k in range(1, p + 1)) != 0
+
+
+def test_fd_reference_degree_mapping():
+    assert [fd.reference_radius(d) for d in (2, 4, 6, 8, 10, 12)] == [1, 2, 3, 4, 4, 4]
+
+
+def test_laplacian_brute_force_small_grid():
+    nx, ny, nz, r = 5, 6, 7, 4
+    a = fd.laplacian(nx, ny, nz, r).toarray()
+    w = [float(c) for c in fd.fd_coefficients(r)]
+    n = nx * ny * nz
+    b = np.zeros((n, n))
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                p = i + nx * (j + ny * k)
+                b[p, p] = 3 * w[0]
+                for d in range(1, r + 1):
+                    for (ii, jj, kk) in [(i + d, j, k), (i - d, j, k), (i, j + d, k), (i, j - d, k), (i, j, k + d),
+                                         (i, j, k - d)]:
+                        if 0 <= ii < nx and 0 <= jj < ny and 0 <= kk < nz:
+                            b[p, ii + nx * (jj + ny * kk)] += w[d]
+    assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------------ splines
+def test_spline_matches_scipy_natural():
+    d = tables.spline_table("O")
+    for col in (tables.COL_CHARGE, tables.COL_POT_S, tables.COL_HARTREE, tables.COL_WFN_P):
+        s = splines.RadialSpline(d[:, 0], d[:, col])
+        cs = CubicSpline(s.xi, s.yi, bc_type="natural")
+        x = np.concatenate([np.linspace(s.xi[0], s.xi[-1], 3001), s.xi])
+        assert np.allclose(s(x), cs(x), rtol=1e-9, atol=1e-12 * np.abs(s.yi).max())
+        # clamping outside the table (fsplevalIO.py:25-30)
+        assert s(np.array([0.0]))[0] == pytest.approx(s.yi[0], rel=1e-12, abs=1e-300)
+        assert s(np.array([1e4]))[0] == pytest.approx(s.yi[-1], rel=1e-12, abs=1e-300)
+
+
+def test_preprocess_rule():
+    v = np.array([0.0, 1e-6, 2e-6, 1.0, 1.0 + 5e-6, 2.0, 2.0])
+    assert list(splines.preprocess(v)) == [0, 3, 5, 6]
+
+
+# ------------------------------------------------------------------ setup vs logs
+@pytest.mark.parametrize("case", ["h2o_sample", "h_smoke"])
+def test_sad_density_and_setup_scalars(case):
+    atoms, dom, nev = _h2o_sample() if case == "h2o_sample" else _h_smoke()
+    mask = grid.mask_flat(dom, "cube")
+    rho0, hpot0, pot, info = pseudo.pseudo_diag(dom, atoms, mask)
+    gold = np.load(os.path.join(GOLD, "h2o_sample_rho_grid.npz" if case == "h2o_sample" else "h_smoke_rho_grid.npz"))
+    assert np.max(np.abs(rho0 - gold["init"])) < 1e-15
+    log = LOGS[case]
+    assert round(info["electron_count_initial"], 6) == log["electron_count_initial"]
+    assert round(float(np.sum(rho0 * hpot0)) * RY_EV, 5) == log["hartree0_ev"]
+    _, exc = xc.exc_nspn(dom["h"], rho0 / dom["h"] ** 3)
+    assert round(exc * RY_EV, 5) == log["exc0_ev"]
+
+
+def test_nuclear_energy_logs():
+    atoms, _, _ = _h2o_sample()
+    assert round(pseudo.nuclear(atoms), 5) == LOGS["h2o_sample"]["e_nn_ry"]
+    atoms, _, _ = M.prepare(M.naphthalene_sample(), h_ang=0.2, radius_ang=9.0)
+    assert round(pseudo.nuclear(atoms), 5) == LOGS["c10h8_cpu"]["e_nn_ry"]
+    assert pseudo.nelectrons(atoms) == 48
+
+
+# ------------------------------------------------------------------ XC
+def test_xc_pz81_closed_form_and_continuity():
+    rho = np.array([0.0, -1e-3, 3 / (4 * np.pi), 1e-4, 0.5])
+    v, e = xc.exc_nspn(1.0, rho)
+    assert v[0] == 0 and v[1] == 0
+    # rs = 1 at rho = 3/(4 pi): both branches give the same e_c (PZ81 continuity to ~1e-4)
+    rs = 1.0
+    ec_hi = -0.2846 / (1 + 1.0529 + 0.3334)
+    ec_lo = 0.0622 * np.log(rs) - 0.096 + (0.004 * np.log(rs) - 0.0232) * rs
+    assert abs(ec_hi - ec_lo) < 2e-4
+    # exchange part: -2/(pi a0 rs) in Ry
+    a0 = (4 / (9 * np.pi)) ** (1 / 3)
+    vx = -2 / (np.pi * a0 * 1.0)
+    vc_hi = ec_hi ** 2 * (1 + 3.5 * 1.0529 / 3 + 4 * 0.3334 / 3) / -0.2846
+    assert v[2] == pytest.approx(vx + vc_hi, rel=1e-14)
+
+
+# ------------------------------------------------------------------ projectors
+def test_projector_tables_match_pointwise_loop():
+    """Factored tables == a direct per-point restatement of pseudoNL_original.py:143-201."""
+    atoms, dom, _ = _h2o_sample()
+    dom = dict(dom)
+    mask = grid.mask_flat(dom, "cube")
+    tabs = pseudo.projector_tables(dom, atoms, mask)
+    assert len(tabs) == 3
+    nx, h, rad = dom["nx"], dom["h"], dom["radius"]
+    sym, xyz = atoms[1]
+    t = tables.spline_table(sym)
+    wfn = splines.RadialSpline(t[:, 0], t[:, tables.COL_WFN_P])
+    pps = splines.RadialSpline(t[:, 0], t[:, tables.COL_POT_P] - t[:, tables.COL_POT_S])
+    R = tables.elements()[sym]["R"]
+    xa, ya, za = xyz[0]
+    i0, j0, k0 = (int(round((q + rad) / h + 1)) for q in (xa, ya, za))
+    span = int(round(R / h))
+    nn, ux = [], []
+    for k in range(max(1, k0 - span), min(nx, k0 + span) + 1):
+        for j in range(max(1, j0 - span), min(nx, j0 + span) + 1):
+            for i in range(max(1, i0 - span), min(nx, i0 + span) + 1):
+                x, y, z = (i - 1) * h - rad - xa, (j - 1) * h - rad - ya, (k - 1) * h - rad - za
+                d = np.sqrt(x * x + y * y + z * z)
+                if 0 < d < R:
+                    nn.append(i - 1 + nx * ((j - 1) + (k - 1) * nx))
+                    ux.append(x / d * wfn(np.array([d]))[0] * pps(np.array([d]))[0])
+    tab = tabs[1]
+    assert np.array_equal(tab["idx"], np.array(nn))
+    assert np.allclose(tab["u"][:, 0], ux, rtol=1e-14, atol=0)
+    assert tab["scale"] == pytest.approx(h ** 3 / tables.elements()[sym]["Zvalue"], rel=1e-15)
+
+
+def test_projector_csr_equals_factored_apply():
+    atoms, dom, _ = _h_smoke()
+    mask = grid.mask_flat(dom, "sphere")
+    tabs = pseudo.projector_tables(dom, atoms, mask)
+    vnl = pseudo.pseudo_nl(dom, atoms, mask)
+    x = np.random.default_rng(0).standard_normal((int(mask.sum()), 3))
+    assert np.allclose(vnl @ x, pseudo.apply_nl(tabs, x), rtol=1e-12, atol=1e-14)
+    assert abs(vnl - vnl.T).max() < 1e-15
+
+
+# ------------------------------------------------------------------ eigensolver pieces
+def test_ch_filter_is_scaled_chebyshev_polynomial():
+    lam = np.linspace(-1.0, 10.0, 40)
+    H = sp.diags(lam)
+    X = np.eye(40)
+    low, high, lam1, deg = 2.0, 10.0, -1.0, 7
+    Y = eig.ch_filter(H, X, deg, lam1, low, high)
+    e, c = (high - low) / 2, (high + low) / 2
+    t = np.polynomial.chebyshev.Chebyshev.basis(deg)
+    expect = t((lam - c) / e) / t((lam1 - c) / e)
+    assert np.allclose(np.diag(Y), expect, rtol=1e-10, atol=1e-12)
+
+
+def test_slim_filter_is_chebyshev_polynomial():
+    lam = np.linspace(-1.0, 10.0, 30)
+    H = sp.diags(lam)
+    Y = eig.cheb_filter_slim(H, np.eye(30), 6, 1.0, 10.0)
+    e, c = 4.5, 5.5
+    t = np.polynomial.chebyshev.Chebyshev.basis(6)
+    assert np.allclose(np.diag(Y), t((c - lam) / e), rtol=1e-10, atol=1e-10)
+
+
+def test_first_filt_and_chefsi1_match_eigsh_small():
+    from scipy.sparse.linalg import eigsh
+    atoms, dom, _ = _h_smoke()
+    dom = dict(dom, nx=12, ny=12, nz=12, radius=0.5 * 0.6 * 11)
+    st = scf.build_setup(atoms, dom, "cube", 8)
+    H = (0.5 * st.A + st.vnl + sp.diags(st.ppot + st.hpot0)).tocsr()
+    rng = np.random.default_rng(3)
+    W, lam, _ = eig.first_filt(12, H, 10, rng)
+    for _ in range(12):
+        W, lam = eig.chefsi1(W, lam, 10, 4, H, rng)
+    ref = np.sort(eigsh(H, k=4, which="SA")[0])
+    assert np.allclose(lam[:4], ref, atol=1e-8)
+    assert np.allclose(W.T @ W, np.eye(12), atol=1e-10)
+
+
+def test_occupations_count_and_fermi_dirac():
+    lam = np.array([-1.0, -0.5, -0.4, 0.1, 0.2, 0.3])
+    ef, occ = eig.occupations(lam, 500.0, 6, 1e-6)
+    assert abs(2 * occ.sum() - 6) <= 1e-6
+    kt = 500.0 * 6.33327186e-06
+    assert np.allclose(occ, 1 / (1 + np.exp((lam - ef) / kt)), rtol=1e-12)
+
+
+# ------------------------------------------------------------------ CG
+def test_pcg_solves_and_reference_aliasing():
+    A = (fd.laplacian(6, 5, 4, 2) * 4.0).tocsr()
+    b = np.random.default_rng(1).standard_normal(A.shape[0])
+    x, its = pcg(A, b, np.zeros_like(b), 500, 1e-12, reference_aliasing=False)
+    assert np.allclose(x, sp.linalg.spsolve(A.tocsc(), b), rtol=1e-9, atol=1e-10)
+    x2, _ = pcg(A, b, np.zeros_like(b), 500, 1e-12, reference_aliasing=True)
+    assert np.allclose(x2, x, rtol=1e-8, atol=1e-9)
+    # the aliasing changes the second direction: after 2 iterations the iterates differ
+    y1, _ = pcg(A, b, np.zeros_like(b), 2, 0.0, reference_aliasing=True)
+    y2, _ = pcg(A, b, np.zeros_like(b), 2, 0.0, reference_aliasing=False)
+    r0 = b.copy()
+    a0 = r0 @ r0 / (r0 @ (A @ r0))
+    x1 = a0 * r0
+    r1 = r0 - a0 * (A @ r0)
+    beta = (r1 @ r1) / (r0 @ r0)
+    p1 = (1 + beta) * r1
+    a1 = (r1 @ r1) / (p1 @ (A @ p1))
+    assert np.allclose(y1, x1 + a1 * p1, rtol=1e-13)
+    assert not np.allclose(y1, y2, rtol=1e-6)
+
+
+# ------------------------------------------------------------------ mixer
+def test_msecant1_broyden_secant_equations_on_linear_problem():
+    rng = np.random.default_rng(2)
+    n = 12
+    J = np.eye(n) * 0.5 + 0.1 * rng.standard_normal((n, n))
+    xs = rng.standard_normal(n)
+    f = lambda x: -J @ (x - xs)  # noqa: E731
+    mx = Msecant1()
+    x = rng.standard_normal(n)
+    hist = []
+    for _ in range(6):
+        fx = f(x)
+        hist.append((x.copy(), fx.copy()))
+        x, m = mx(x, fx)
+    # first step is simple mixing
+    assert np.allclose(hist[1][0], hist[0][0] + 0.5 * hist[0][1])
+    # converges towards the root on a linear problem
+    assert np.linalg.norm(f(x)) < np.linalg.norm(hist[0][1])
+
+
+def test_msecant1_restart_rule():
+    mx = Msecant1()
+    x0, f0 = np.ones(3), np.full(3, 1e-3)
+    mx(x0, f0)
+    mx(x0 + 1, f0 * 2)
+    xr, m = mx(x0 + 2, np.full(3, 10.0))  # |f_prev| < 0.1 |f_new| -> restart from the stored pair
+    assert m == 0 and np.allclose(xr, (x0 + 1) + 0.5 * f0 * 2)
+
+
+# ------------------------------------------------------------------ end to end
+@pytest.mark.slow
+def test_h2o_sample_scf_reproduces_reference_log():
+    atoms, dom, nev = _h2o_sample()
+    st = scf.build_setup(atoms, dom, "cube", 8)
+    r = scf.run_scf(st, nev, tol=2e-4, maxits=50, poldeg=10, seed=1)
+    log = LOGS["h2o_sample"]
+    assert r.converged and abs(r.iterations - log["iterations"]) <= 2
+    # the reference's own last step changed E by 5e-3 Ry at this loose tolerance
+    assert abs(r.energies["total_ry"] - log["e_total_ry"]) < 2e-3
+    assert np.allclose(2 * r.eigenvalues[:4], log["occupied_2lambda_ry"], atol=5e-5)
+    gold = np.load(os.path.join(GOLD, "h2o_sample_rho_grid.npz"))
+    assert np.max(np.abs(r.rho * dom["h"] ** 3 - gold["conv"])) < 5e-6
