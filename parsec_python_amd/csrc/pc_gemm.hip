@@ -8,8 +8,9 @@
 // rotates with W @ Q (chefsi1.py:67, chefsi1_gpu.py:66).
 //
 // MFMA 16x16x4 (f64: v_mfma_f64_16x16x4f64, f32: v_mfma_f32_16x16x4f32) lane
-// layout: A[m][k] from lane m + 16k, B[k][n] from lane n + 16k, D[m][n] in
-// lane n + 16*(m/4), register m%4.  The K index of an operand is only a label
+// layout on gfx950 (measured with one-hot operands): A[m][k] from lane m + 16k,
+// B[k][n] from lane n + 16k; D[m][n] in lane n + 16*(m/4), register m%4 for
+// f32, and in lane n + 16*(m%4), register m/4 for f64.  The K index of an operand is only a label
 // that A and B must share, so each lane loads 4 consecutive points (one
 // 16-byte/32-byte load) and step s of 4 uses point 4*(lane/16)+s.
 //
@@ -30,12 +31,15 @@ template <> struct Acc<double> {
   static __device__ __forceinline__ v4d mfma(double a, double b, v4d c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // output row of register r in lane group lk (= lane/16)
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * r + lk; }
 };
 template <> struct Acc<float> {
   typedef v4f type;
   static __device__ __forceinline__ v4f mfma(float a, float b, v4f c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int lk, int r) { return 4 * lk + r; }
 };
 
 // Load 4 consecutive points [p, p+4) of column col; zero beyond n.
@@ -127,7 +131,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const T* __restrict__ X, long
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int ii = wi * 32 + 16 * ta + 4 * lk + r;
+        const int ii = wi * 32 + 16 * ta + Acc<T>::row(lk, r);
         const int jj = wj * 32 + 16 * tb + lm;
         out[jj * 64 + ii] = (double)acc[ta][tb][r];
       }
@@ -195,7 +199,7 @@ __global__ __launch_bounds__(256) void rotate_kernel(const T* __restrict__ X, lo
     for (int tb = 0; tb < 2; ++tb)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int col = c0 + 16 * ta + 4 * lk + r;
+        const int col = c0 + 16 * ta + Acc<T>::row(lk, r);
         const long long p = p0 + 16 * tb + lm;
         if (col < ky && p < n) Z[(long long)col * ldz + p] = acc[ta][tb][r];
       }

@@ -91,7 +91,9 @@ def pseudo_diag(grid: Grid, dgrid: DeviceGrid, atoms):
         t = tables.table(s)
         for col in (tables.CHARGE, tables.POT_S, tables.HARTREE):
             sp = Spline(t[:, tables.RADIUS], t[:, col])
-            xi_all.append(sp.xi); z_all.append(sp.z); c_all.append(sp.c); d_all.append(sp.d)
+            # c, d have one entry per interval (n-1); pad to the knot count so one offset serves all four
+            xi_all.append(sp.xi); z_all.append(sp.z)
+            c_all.append(np.append(sp.c, 0.0)); d_all.append(np.append(sp.d, 0.0))
             off.append(cur); ln.append(len(sp.xi)); cur += len(sp.xi)
     pos, spec = [], []
     zsum = 0.0
