@@ -15,8 +15,9 @@
 // 16-byte/32-byte load) and step s of 4 uses point 4*(lane/16)+s.
 //
 // gram splits the n points into chunks; each workgroup accumulates one output
-// tile over one chunk in FP32 (f32 inputs) or FP64 MFMA and writes an FP64
-// partial; a second kernel sums the partials in a fixed order (deterministic).
+// tile over one chunk (FP64 MFMA, or FP32 MFMA per 16 points flushed into FP64
+// registers) and writes an FP64 partial; a second kernel sums the partials in a
+// fixed order (deterministic).
 #include "pc_common.h"
 #include <algorithm>
 
@@ -101,11 +102,19 @@ __global__ __launch_bounds__(256) void gram_kernel(const T* __restrict__ X, long
   const int wi = w >> 1, wj = w & 1;
   const int i0 = ti * 64 + wi * 32, j0 = tj * 64 + wj * 32;
   const int lm = lane & 15, lk = lane >> 4;
+  // FP32 inputs: the MFMA accumulates 16 points in FP32, then the block is
+  // added into FP64 accumulators (products of FP32 values are exact in FP64,
+  // so the only FP32 rounding is within one 16-term block).
   AT acc[2][2];
+  double acc64[2][2][4];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int c = 0; c < 2; ++c) acc[a][c] = AT{0, 0, 0, 0};
+    for (int c = 0; c < 2; ++c) {
+      acc[a][c] = AT{0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[a][c][r] = 0.0;
+    }
   const long long pbeg = (long long)ch * chunk;
   const long long pend = min(n, pbeg + chunk);
   for (long long p = pbeg; p < pend; p += 16) {
@@ -122,6 +131,16 @@ __global__ __launch_bounds__(256) void gram_kernel(const T* __restrict__ X, long
       for (int ta = 0; ta < 2; ++ta)
 #pragma unroll
         for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = Acc<T>::mfma(a[ta][s], bb[tb][s], acc[ta][tb]);
+    if constexpr (sizeof(T) == 4) {
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc64[ta][tb][r] += (double)acc[ta][tb][r];
+          acc[ta][tb] = AT{0, 0, 0, 0};
+        }
+    }
   }
   // partial layout: part[(ch*ntile + tile)*4096 + jj*64 + ii], tile-local 64x64 col-major
   double* out = part + ((long long)ch * ntile + tile) * 4096;
@@ -133,7 +152,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const T* __restrict__ X, long
       for (int r = 0; r < 4; ++r) {
         const int ii = wi * 32 + 16 * ta + Acc<T>::row(lk, r);
         const int jj = wj * 32 + 16 * tb + lm;
-        out[jj * 64 + ii] = (double)acc[ta][tb][r];
+        out[jj * 64 + ii] = (sizeof(T) == 4) ? acc64[ta][tb][r] : (double)acc[ta][tb][r];
       }
 }
 

@@ -54,6 +54,11 @@ class SCFConfig:
     pcg_aliasing: bool = True
     pcg_maxit: int = 200
     pcg_tol: float = 1e-5
+    # mixed precision: switch the wave functions to FP64 once |dE| falls below
+    # polish_de Ha/atom (or err below polish_err) and finish in FP64
+    polish: bool = True
+    polish_de: float = 2e-5
+    polish_err: float | None = None
     verbose: bool = False
 
 
@@ -144,6 +149,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     pot_next = torch.empty(n, **dev)
     occ2_dev = torch.empty(k, **dev)
 
+    H.dtype = prob.dtype
     eig = Eigensolver(H, k, seed=cfg.seed)
     hart = HartreeSolver(H.dgrid, h, reference_aliasing=cfg.pcg_aliasing)
     mixer = DeviceMsecant1(n, cfg.maxits + 2)
@@ -201,6 +207,22 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         if cfg.verbose:
             print(f"  SCF {its:3d}  err {err:9.2e}  E {e['total_ry']:.8f} Ry  dE {de if de is not None else 0:.2e}"
                   f" Ha/atom  CG {cg_its}")
+        if H.dtype == F32 and cfg.polish and ((de is not None and de < cfg.polish_de) or
+                                              (cfg.polish_err is not None and err < cfg.polish_err)):
+            # finish in FP64: convert the current Ritz block and keep iterating
+            H.dtype = F64
+            eig64 = Eigensolver(H, k, seed=cfg.seed)
+            eig64.seeds = eig.seeds
+            _lib.call("pc_convert_f32_to_f64", n, k, ptr(W), W.shape[1], ptr(eig64.bufs[0]), eig64.bufs[0].shape[1],
+                      stream())
+            W = eig64.bufs[0]
+            eig.close()
+            eig64.stats["filter_steps"] += eig.stats["filter_steps"]
+            eig64.stats["rr"] += eig.stats["rr"]
+            eig = eig64
+            hist[-1]["switch_to_fp64"] = True
+            e_prev = None  # the convergence test restarts on FP64 iterates
+            continue
         if cfg.etol_ha_per_atom is not None and de is not None and de < cfg.etol_ha_per_atom:
             converged = True
             break
