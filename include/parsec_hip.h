@@ -132,7 +132,8 @@ int pc_rand_f32(long long n, int ncol, long long ld, unsigned long long seed, in
  * replaces: src/Eigensolvers/pcg.py:7-85 (CPU) / pcg_gpu.py:22-65 (CuPy).
  * Solves (-lap/h^2) x = rhs, FP64, warm start x, stop when r.r <= tol^2 r0.r0
  * or maxit; alias_first=1 reproduces the CPU reference's first-iteration p/r
- * aliasing.  Synchronises every check_every iterations. */
+ * aliasing.  Synchronises every check_every iterations.  res_out (host, may be
+ * NULL) receives {final r.r, tol^2 * r0.r0}. */
 int pc_pcg_workspace(void* grid, size_t* out_bytes);
 int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
                  int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream);

@@ -34,6 +34,7 @@ struct GridDev {
   int* hi;       // [ny*nz]
   int* off;      // [ny*nz]
   int* zrange;   // [2*ny]: min z, max z+1 of non-empty rows per y
+  int2* meta;    // [ny*nz]: {off - lo, lo | hi << 16}
   int radius;
   double w[8];   // 1D -d2/dx2 weights, centre first (unscaled)
 };
@@ -44,6 +45,7 @@ struct StArgs {
   const int* hi;
   const int* off;
   const int* zrange;
+  const int2* meta;
   long long ld;
   int nslab;
   int nvec;
@@ -57,6 +59,24 @@ __device__ __forceinline__ T row_val(const T* __restrict__ y, int lo, int hi, in
   return (i >= lo && i < hi) ? y[off + (i - lo)] : T(0);
 }
 
+// LDS-only workgroup barrier: waits for this wave's LDS traffic, not for
+// outstanding global loads (the prefetch of the next plane stays in flight).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0); vmcnt/expcnt left at max
+  __builtin_amdgcn_s_barrier();
+}
+
+template <typename T>
+__device__ __forceinline__ T meta_val(const T* __restrict__ y, int2 m, int i) {
+  const int lo = m.y & 0xffff, hi = m.y >> 16;
+  return (i >= lo && i < hi) ? y[m.x + i] : T(0);
+}
+
+// Software-pipelined z-march: while plane z is computed from LDS and the
+// register queues, the loads for plane z+1 (queue entry z+1+R, halo rows,
+// V and x) are already in flight.  Row metadata {off-lo, lo|hi<<16} is read
+// with wave-uniform (scalar) loads.  Two LDS tiles alternate, so one barrier
+// per plane suffices.
 template <typename T, int R, int SX, int TYW>
 __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
                                                       T* Out, const T* __restrict__ V, T ca, T cb, T cc,
@@ -66,7 +86,8 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   constexpr int NXP = 64 * SX;
   constexpr int LW = NXP + 2 * R;   // LDS row stride (R zero columns each side)
   constexpr int LR = TY + 2 * R;    // LDS rows
-  __shared__ T tile[LR * LW];
+  constexpr int HT = (2 * R + NW - 1) / NW;  // halo rows per wave
+  __shared__ T tile[2][LR * LW];
   __shared__ double red[NW];
 
   const int bid = blockIdx.x;
@@ -76,17 +97,18 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   const int slab = L - v * g.nslab;
   const int j0 = slab * TY;
   const int lane = threadIdx.x & 63;
-  const int wv = threadIdx.x >> 6;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ny = g.ny, nz = g.nz;
+  const int2* __restrict__ meta = g.meta;
 
   const T* __restrict__ yv = Y + (long long)v * g.ld;
   const T* xv = X ? X + (long long)v * g.ld : nullptr;
   T* ov = Out + (long long)v * g.ld;
 
-  // z-range of this slab's output rows.
-  int zb = g.nz, ze = 0;
+  int zb = nz, ze = 0;
   for (int t = 0; t < TY; ++t) {
     const int j = j0 + t;
-    if (j < g.ny) {
+    if (j < ny) {
       zb = min(zb, g.zrange[2 * j]);
       ze = max(ze, g.zrange[2 * j + 1]);
     }
@@ -94,18 +116,24 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   double dot = 0.0;
   if (zb >= ze) {
     if (dot_part) {
-      double s = pc_block_sum(0.0, red);
+      const double s = pc_block_sum(0.0, red);
       if (threadIdx.x == 0) dot_part[L] = s;
     }
     return;
   }
-
-  // zero the LDS (pad columns stay zero for the whole launch)
-  for (int t = threadIdx.x; t < LR * LW; t += 256) tile[t] = T(0);
+  for (int t = threadIdx.x; t < 2 * LR * LW; t += 256) (&tile[0][0])[t] = T(0);
   __syncthreads();
 
+  auto rowmeta = [&](int j, int z) -> int2 {
+    return (j >= 0 && j < ny && z >= 0 && z < nz) ? meta[j + ny * z] : make_int2(0, 0);
+  };
+
   T q[TYW][SX][2 * R + 1];
-  // prefill planes zb-R .. zb+R-1 into q[..][..][1..2R]
+  T nq[TYW][SX], vq[TYW][SX], xq[TYW][SX], nv[TYW][SX], nxv[TYW][SX];
+  T hc[HT][SX] = {}, hn[HT][SX] = {};
+  int2 mcur[TYW], mnext[TYW];
+
+  // prefill planes zb-R .. zb+R-1
 #pragma unroll
   for (int r = 0; r < TYW; ++r) {
     const int j = j0 + wv + NW * r;
@@ -113,87 +141,109 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
     for (int s = 0; s < SX; ++s) q[r][s][0] = T(0);
 #pragma unroll
     for (int dz = 1; dz <= 2 * R; ++dz) {
-      const int z = zb - R - 1 + dz;
-      int lo = 0, hi = 0, off = 0;
-      if (j < g.ny && z >= 0 && z < g.nz) {
-        const int row = j + g.ny * z;
-        lo = g.lo[row]; hi = g.hi[row]; off = g.off[row];
-      }
+      const int2 m = rowmeta(j, zb - R - 1 + dz);
 #pragma unroll
-      for (int s = 0; s < SX; ++s) q[r][s][dz] = row_val(yv, lo, hi, off, lane + 64 * s);
+      for (int s = 0; s < SX; ++s) q[r][s][dz] = meta_val(yv, m, lane + 64 * s);
     }
   }
-
-  for (int z = zb; z < ze; ++z) {
-    // (1) shift the z-queues and load plane z+R
+  // prefetch everything plane z needs, for z = zb
+  auto prefetch = [&](int z) {
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
       const int j = j0 + wv + NW * r;
-      const int zz = z + R;
-      int lo = 0, hi = 0, off = 0;
-      if (j < g.ny && zz < g.nz) {
-        const int row = j + g.ny * zz;
-        lo = g.lo[row]; hi = g.hi[row]; off = g.off[row];
+      const int2 mq = rowmeta(j, z + R);
+      const int2 mo = rowmeta(j, z);
+      mnext[r] = mo;
+#pragma unroll
+      for (int s = 0; s < SX; ++s) {
+        const int i = lane + 64 * s;
+        nq[r][s] = meta_val(yv, mq, i);
+        const int lo = mo.y & 0xffff, hi = mo.y >> 16;
+        const bool in = i >= lo && i < hi;
+        nv[r][s] = in ? V[mo.x + i] : T(0);
+        nxv[r][s] = (in && xv) ? xv[mo.x + i] : T(0);
       }
+    }
+#pragma unroll
+    for (int t = 0; t < HT; ++t) {
+      const int hr = wv + NW * t;
+      if (hr < 2 * R) {
+        const int tr = hr < R ? hr : hr + TY;
+        const int2 mh = rowmeta(j0 - R + tr, z);
+#pragma unroll
+        for (int s = 0; s < SX; ++s) hn[t][s] = meta_val(yv, mh, lane + 64 * s);
+      }
+    }
+  };
+  prefetch(zb);
+
+  for (int z = zb; z < ze; ++z) {
+    const int b = (z - zb) & 1;
+    T* tb = tile[b];
+    // commit the prefetched plane
+#pragma unroll
+    for (int r = 0; r < TYW; ++r) {
+      mcur[r] = mnext[r];
 #pragma unroll
       for (int s = 0; s < SX; ++s) {
 #pragma unroll
         for (int d = 0; d < 2 * R; ++d) q[r][s][d] = q[r][s][d + 1];
-        q[r][s][2 * R] = row_val(yv, lo, hi, off, lane + 64 * s);
+        q[r][s][2 * R] = nq[r][s];
+        vq[r][s] = nv[r][s];
+        xq[r][s] = nxv[r][s];
       }
     }
-    // (2) plane z into LDS: interior rows from the queues, halo rows from memory
+#pragma unroll
+    for (int t = 0; t < HT; ++t)
+#pragma unroll
+      for (int s = 0; s < SX; ++s) hc[t][s] = hn[t][s];
+    // issue the loads for the next plane
+    if (z + 1 < ze) prefetch(z + 1);
+    // plane z into LDS
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
       const int tr = wv + NW * r + R;
 #pragma unroll
-      for (int s = 0; s < SX; ++s) tile[tr * LW + R + lane + 64 * s] = q[r][s][R];
+      for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = q[r][s][R];
     }
-    for (int hr = wv; hr < 2 * R; hr += NW) {
-      const int tr = hr < R ? hr : hr + TY;
-      const int j = j0 - R + tr;
-      int lo = 0, hi = 0, off = 0;
-      if (j >= 0 && j < g.ny) {
-        const int row = j + g.ny * z;
-        lo = g.lo[row]; hi = g.hi[row]; off = g.off[row];
-      }
 #pragma unroll
-      for (int s = 0; s < SX; ++s) tile[tr * LW + R + lane + 64 * s] = row_val(yv, lo, hi, off, lane + 64 * s);
+    for (int t = 0; t < HT; ++t) {
+      const int hr = wv + NW * t;
+      if (hr < 2 * R) {
+        const int tr = hr < R ? hr : hr + TY;
+#pragma unroll
+        for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = hc[t][s];
+      }
     }
-    __syncthreads();
-    // (3) outputs
+    lds_barrier();
+    // outputs of plane z
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
-      const int j = j0 + wv + NW * r;
-      if (j < g.ny) {
-        const int row = j + g.ny * z;
-        const int lo = g.lo[row], hi = g.hi[row], off = g.off[row];
-        const int tr = wv + NW * r + R;
+      const int2 mo = mcur[r];
+      const int lo = mo.y & 0xffff, hi = mo.y >> 16;
+      const int tr = wv + NW * r + R;
 #pragma unroll
-        for (int s = 0; s < SX; ++s) {
-          const int i = lane + 64 * s;
-          if (i >= lo && i < hi) {
-            const T yc = q[r][s][R];
-            T acc = T(g.w[0]) * yc;
-            const T* trow = tile + tr * LW + R + i;
+      for (int s = 0; s < SX; ++s) {
+        const int i = lane + 64 * s;
+        if (i >= lo && i < hi) {
+          const T yc = q[r][s][R];
+          T acc = T(g.w[0]) * yc;
+          const T* trow = tb + tr * LW + R + i;
 #pragma unroll
-            for (int d = 1; d <= R; ++d) {
-              const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[r][s][R - d] + q[r][s][R + d];
-              acc += T(g.w[d]) * nb;
-            }
-            const long long p = off + (i - lo);
-            T o = acc + (ca * V[p] + cb) * yc;
-            if (xv) o += cc * xv[p];
-            ov[p] = o;
-            if (dot_part) dot += (double)o * (double)yc;
+          for (int d = 1; d <= R; ++d) {
+            const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[r][s][R - d] + q[r][s][R + d];
+            acc += T(g.w[d]) * nb;
           }
+          T o = acc + (ca * vq[r][s] + cb) * yc;
+          if (xv) o += cc * xq[r][s];
+          ov[mo.x + i] = o;
+          if (dot_part) dot += (double)o * (double)yc;
         }
       }
     }
-    __syncthreads();
   }
   if (dot_part) {
-    double s = pc_block_sum(dot, red);
+    const double s = pc_block_sum(dot, red);
     if (threadIdx.x == 0) dot_part[L] = s;
   }
 }
@@ -205,7 +255,7 @@ int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
   constexpr int TY = 4 * TYW;
   StArgs a;
   a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
-  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange;
+  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
   a.ld = ld;
   a.nslab = (gd->ny + TY - 1) / TY;
   a.nvec = nvec;
@@ -310,7 +360,7 @@ int launch_direct_r(const GridDev* gd, const T* Y, const T* X, T* Out, const T* 
                     double scale, double ca, double cb, double cc, double* dot_part, hipStream_t st) {
   StArgs a;
   a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
-  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange;
+  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
   a.ld = ld; a.nslab = 0; a.nvec = nvec; a.nblocks = 0; a.nb8 = 0;
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
@@ -407,6 +457,8 @@ int pc_grid_create(int nx, int ny, int nz, const int* row_lo, const int* row_hi,
   std::vector<int> lo(row_lo, row_lo + nrow), hi(row_hi, row_hi + nrow);
   for (int r = 0; r < nrow; ++r)
     if (hi[r] <= lo[r]) { lo[r] = 0; hi[r] = 0; }
+  std::vector<int2> meta(nrow);
+  for (int r = 0; r < nrow; ++r) meta[r] = make_int2(off[r] - lo[r], lo[r] | (hi[r] << 16));
   GridDev* g = new GridDev();
   g->nx = nx; g->ny = ny; g->nz = nz; g->npts = cnt; g->radius = radius;
   for (int d = 0; d < 8; ++d) g->w[d] = d <= radius ? weights[d] : 0.0;
@@ -415,6 +467,8 @@ int pc_grid_create(int nx, int ny, int nz, const int* row_lo, const int* row_hi,
   if (e == hipSuccess) e = hipMalloc(&g->hi, nrow * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&g->off, nrow * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&g->zrange, 2 * ny * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&g->meta, nrow * sizeof(int2));
+  if (e == hipSuccess) e = hipMemcpy(g->meta, meta.data(), nrow * sizeof(int2), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->lo, lo.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->hi, hi.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->off, off.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
@@ -432,6 +486,7 @@ int pc_grid_destroy(void* grid) {
   GridDev* g = static_cast<GridDev*>(grid);
   if (!g) return pc::PC_OK;
   (void)hipFree(g->lo); (void)hipFree(g->hi); (void)hipFree(g->off); (void)hipFree(g->zrange);
+  (void)hipFree(g->meta);
   delete g;
   return pc::PC_OK;
 }

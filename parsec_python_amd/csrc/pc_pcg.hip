@@ -122,7 +122,8 @@ int pc_pcg_workspace(void* grid, size_t* out_bytes) {
 }
 
 // x (in: warm start, out: solution) and rhs are device FP64 arrays of npts.
-// inv_h2 = 1/h^2.  its_out receives the iteration count (host).
+// inv_h2 = 1/h^2.  its_out receives the iteration count (host); res_out (host,
+// optional, 2 doubles) receives the final r.r and tol^2 * r0.r0.
 int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
                  int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream) {
   PC_CHECK_ARG(grid && rhs && x && work && its_out, "null pointer");
@@ -184,7 +185,10 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
       status = pc::PC_ERR_HIP;
     } else {
       *its_out = (int)host[S_ITS];
-      if (res_out) *res_out = host[S_RO];
+      if (res_out) {
+        res_out[0] = host[S_RO];
+        res_out[1] = host[S_TOL1];
+      }
     }
   }
   return status;

@@ -229,6 +229,7 @@ class Eigensolver:
         lam1 = float(np.min(ritzv))
         if lowerb > upperb:
             raise ValueError("Bounds are incorrect")
+        self.last_bounds = (lam1, lowerb, upperb)
         Y = self.ch_filter(W, k, deg, lam1, lowerb, upperb)
         return self.rayleigh_ritz(Y, k)
 

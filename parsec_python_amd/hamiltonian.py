@@ -46,6 +46,9 @@ class Hamiltonian:
         self.pot = None
         self.pot32 = None
         self.set_potential(pot)
+        # optional per-launch timing of the multi-vector stencil: list of
+        # (start event, end event, algorithmic bytes)
+        self.stencil_timer: list | None = None
 
     # ------------------------------------------------------------------ potential
     def set_potential(self, pot: torch.Tensor) -> None:
@@ -83,9 +86,19 @@ class Hamiltonian:
         if self.proj is not None and self.proj.natoms > 0:
             cbuf = self.scratch.get("projC", 3 * self.proj.natoms * nvec)
             _lib.call(f"pc_proj_gather_{suffix}", self.proj.handle, ptr(Y), ld, nvec, ptr(cbuf), st)
-        kernel = f"pc_stencil_{suffix}" if nvec >= ZMARCH_MIN_VECTORS else f"pc_stencil_direct_{suffix}"
+        multi = nvec >= ZMARCH_MIN_VECTORS
+        kernel = f"pc_stencil_{suffix}" if multi else f"pc_stencil_direct_{suffix}"
+        timed = multi and self.stencil_timer is not None
+        if timed:
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
         _lib.call(kernel, self.dgrid.handle, ptr(Y), ptr(X) if X is not None else C.c_void_p(0), ptr(out), ptr(V),
                   nvec, ld, a * self.kin_scale, a, b, c, C.c_void_p(0), st)
+        if timed:
+            e1.record()
+            esz = 8 if dt == F64 else 4
+            self.stencil_timer.append((e0, e1, (3 if X is not None else 2) * self.n * nvec * esz))
         if cbuf is not None:
             _lib.call(f"pc_proj_scatter_{suffix}", self.proj.handle, ptr(cbuf), ptr(out), ld, nvec, a, st)
 

@@ -34,9 +34,11 @@ class HartreeSolver:
         """x (warm start, overwritten with the solution); returns iterations."""
         work = self.scratch.bytes("pcg", self.work_bytes)
         its = C.c_int()
-        res = C.c_double()
+        res = (C.c_double * 2)()
         _lib.call("pc_pcg_solve", self.dgrid.handle, self.inv_h2, ptr(rhs), ptr(x), maxit, tol, self.alias,
-                  self.check_every, ptr(work), self.work_bytes, C.byref(its), C.byref(res), stream())
+                  self.check_every, ptr(work), self.work_bytes, C.byref(its), res, stream())
         self.last_its = its.value
-        self.last_res = res.value
+        self.last_res = res[0]
+        # relative residual |r|/|r0| reached (tol^2 r0.r0 was stored)
+        self.last_rel = (res[0] * tol * tol / res[1]) ** 0.5 if res[1] > 0 else 0.0
         return its.value

@@ -194,6 +194,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         de = None if e_prev is None else abs(e_ha_atom - e_prev)
         e_prev = e_ha_atom
         hist.append({"iter": its, "err": err, "total_ry": e["total_ry"], "dE_ha_atom": de, "cg_its": cg_its,
+                     "cg_rel": hart.last_rel, "bounds": getattr(eig, "last_bounds", None),
                      "lam": np.array(lam[:nev])})
         # pot <- mixer(pot, pot_new - pot)
         f1.copy_(pot_new)
@@ -205,8 +206,10 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         tim["hartree"] += t2 - t1
         tim["post"] += t3 - t2
         if cfg.verbose:
+            b = getattr(eig, "last_bounds", None)
             print(f"  SCF {its:3d}  err {err:9.2e}  E {e['total_ry']:.8f} Ry  dE {de if de is not None else 0:.2e}"
-                  f" Ha/atom  CG {cg_its}")
+                  f" Ha/atom  CG {cg_its} rel {hart.last_rel:.1e}"
+                  + (f"  bounds {b[0]:.4f} {b[1]:.4f} {b[2]:.3f}" if b else ""))
         if H.dtype == F32 and cfg.polish and ((de is not None and de < cfg.polish_de) or
                                               (cfg.polish_err is not None and err < cfg.polish_err)):
             # finish in FP64: convert the current Ritz block and keep iterating
