@@ -21,6 +21,7 @@ _vp = C.c_void_p
 _i = C.c_int
 _ll = C.c_longlong
 _d = C.c_double
+_f = C.c_float
 _sz = C.c_size_t
 _u64 = C.c_ulonglong
 _pi = C.POINTER(C.c_int)
@@ -74,18 +75,20 @@ PROTOTYPES = {
     "pc_sad_eval": [_vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pc_spline_eval": [_vp, _vp, _vp, _vp, _i, _vp, _ll, _vp, _vp],
     # learned initialiser (3D U-Net) kernels
-    "pc_conv3d_fwd": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp],
-    "pc_conv3d_bwd_data": [_vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp],
-    "pc_conv3d_bwd_weight": [_vp, _vp, _vp, _vp, _i, _i, _i, _i, _i, _i, _i, _vp, _sz, _vp],
-    "pc_act_fwd": [_vp, _vp, _ll, _i, _vp],
-    "pc_act_bwd": [_vp, _vp, _vp, _ll, _i, _vp],
-    "pc_pool2_fwd": [_vp, _vp, _i, _i, _i, _i, _i, _vp],
-    "pc_pool2_bwd": [_vp, _vp, _i, _i, _i, _i, _i, _vp],
-    "pc_upsample2_fwd": [_vp, _vp, _i, _i, _i, _i, _i, _vp],
-    "pc_upsample2_bwd": [_vp, _vp, _i, _i, _i, _i, _i, _vp],
-    "pc_adam_step": [_vp, _vp, _vp, _vp, _vp, _ll, _d, _d, _d, _d, _d, _i, _vp],
-    "pc_cast_bf16": [_vp, _vp, _ll, _vp],
-    "pc_mse_grad": [_vp, _vp, _vp, _vp, _ll, _d, _vp, _vp],
+    "pc_unet_conv3d_fwd": [_vp, _i, _i, _vp, _vp, _i, _i, _i, _i, _i, _f, _vp, _i, _vp],
+    "pc_unet_conv3d_wflip": [_vp, _i, _i, _vp, _vp],
+    "pc_unet_wgrad_workspace": [_i, _i, _psz],
+    "pc_unet_conv3d_wgrad": [_vp, _i, _i, _vp, _i, _i, _i, _i, _i, _vp, _vp, _sz, _vp],
+    "pc_unet_act_bwd_workspace": [_i, _ll, _psz],
+    "pc_unet_act_bwd": [_vp, _i, _vp, _i, _i, _i, _i, _i, _f, _vp, _i, _vp, _vp, _sz, _vp],
+    "pc_unet_pool2_fwd": [_vp, _i, _i, _i, _i, _i, _vp, _i, _vp],
+    "pc_unet_pool2_bwd": [_vp, _i, _vp, _i, _i, _i, _i, _i, _vp, _i, _vp],
+    "pc_unet_up2_fwd": [_vp, _i, _i, _i, _i, _i, _vp, _i, _vp],
+    "pc_unet_up2_bwd": [_vp, _i, _i, _i, _i, _i, _vp, _i, _vp],
+    "pc_unet_cast_bf16": [_vp, _ll, _vp, _vp],
+    "pc_unet_adam": [_vp, _vp, _vp, _vp, _ll, _d, _d, _d, _d, _d, _i, _vp, _vp],
+    "pc_unet_mse_workspace": [_psz],
+    "pc_unet_mse": [_vp, _vp, _ll, _vp, _vp, _vp, _sz, _vp],
 }
 
 _RESTYPES = {"pc_last_error": C.c_char_p}

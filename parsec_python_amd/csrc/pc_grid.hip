@@ -24,6 +24,7 @@
 #include "pc_common.h"
 #include <vector>
 #include <algorithm>
+#include <cstdlib>
 
 namespace {
 
@@ -35,6 +36,7 @@ struct GridDev {
   int* off;      // [ny*nz]
   int* zrange;   // [2*ny]: min z, max z+1 of non-empty rows per y
   int2* meta;    // [ny*nz]: {off - lo, lo | hi << 16}
+  int* prow;     // [npts]: row (j + ny*k) of each domain point
   int radius;
   double w[8];   // 1D -d2/dx2 weights, centre first (unscaled)
 };
@@ -74,10 +76,10 @@ __device__ __forceinline__ T meta_val(const T* __restrict__ y, int2 m, int i) {
 
 // Software-pipelined z-march: while plane z is computed from LDS and the
 // register queues, the loads for plane z+1 (queue entry z+1+R, halo rows,
-// V and x) are already in flight.  Row metadata {off-lo, lo|hi<<16} is read
-// with wave-uniform (scalar) loads.  Two LDS tiles alternate, so one barrier
-// per plane suffices.
-template <typename T, int R, int SX, int TYW>
+// V and x) are already in flight.  A workgroup handles NV vectors at once:
+// the row metadata, the V values and the control flow are shared by them.
+// Two LDS tiles alternate, so one barrier per plane suffices.
+template <typename T, int R, int SX, int TYW, int NV>
 __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
                                                       T* Out, const T* __restrict__ V, T ca, T cb, T cc,
                                                       double* __restrict__ dot_part) {
@@ -87,23 +89,34 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   constexpr int LW = NXP + 2 * R;   // LDS row stride (R zero columns each side)
   constexpr int LR = TY + 2 * R;    // LDS rows
   constexpr int HT = (2 * R + NW - 1) / NW;  // halo rows per wave
-  __shared__ T tile[2][LR * LW];
+  __shared__ T tile[2][NV][LR * LW];
   __shared__ double red[NW];
 
   const int bid = blockIdx.x;
   const int L = (bid & 7) * g.nb8 + (bid >> 3);
   if (L >= g.nblocks) return;
-  const int v = L / g.nslab;
-  const int slab = L - v * g.nslab;
+  const int vg = L / g.nslab;
+  const int slab = L - vg * g.nslab;
   const int j0 = slab * TY;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ny = g.ny, nz = g.nz;
   const int2* __restrict__ meta = g.meta;
 
-  const T* __restrict__ yv = Y + (long long)v * g.ld;
-  const T* xv = X ? X + (long long)v * g.ld : nullptr;
-  T* ov = Out + (long long)v * g.ld;
+  const T* yv[NV];
+  const T* xv[NV];
+  T* ov[NV];
+  bool act[NV];
+#pragma unroll
+  for (int u = 0; u < NV; ++u) {
+    const int v = vg * NV + u;
+    act[u] = v < g.nvec;
+    const long long base = (long long)(act[u] ? v : vg * NV) * g.ld;
+    yv[u] = Y + base;
+    xv[u] = X ? X + base : nullptr;
+    ov[u] = Out + base;
+  }
+  const bool hasx = X != nullptr;
 
   int zb = nz, ze = 0;
   for (int t = 0; t < TY; ++t) {
@@ -121,16 +134,17 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
     }
     return;
   }
-  for (int t = threadIdx.x; t < 2 * LR * LW; t += 256) (&tile[0][0])[t] = T(0);
+  for (int t = threadIdx.x; t < 2 * NV * LR * LW; t += 256) (&tile[0][0][0])[t] = T(0);
   __syncthreads();
 
   auto rowmeta = [&](int j, int z) -> int2 {
     return (j >= 0 && j < ny && z >= 0 && z < nz) ? meta[j + ny * z] : make_int2(0, 0);
   };
 
-  T q[TYW][SX][2 * R + 1];
-  T nq[TYW][SX], vq[TYW][SX], xq[TYW][SX], nv[TYW][SX], nxv[TYW][SX];
-  T hc[HT][SX] = {}, hn[HT][SX] = {};
+  T q[NV][TYW][SX][2 * R + 1];
+  T nq[NV][TYW][SX], xq[NV][TYW][SX], nxv[NV][TYW][SX];
+  T vq[TYW][SX], nv[TYW][SX];
+  T hc[NV][HT][SX] = {}, hn[NV][HT][SX] = {};
   int2 mcur[TYW], mnext[TYW];
 
   // prefill planes zb-R .. zb+R-1
@@ -138,12 +152,16 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   for (int r = 0; r < TYW; ++r) {
     const int j = j0 + wv + NW * r;
 #pragma unroll
-    for (int s = 0; s < SX; ++s) q[r][s][0] = T(0);
+    for (int u = 0; u < NV; ++u)
+#pragma unroll
+      for (int s = 0; s < SX; ++s) q[u][r][s][0] = T(0);
 #pragma unroll
     for (int dz = 1; dz <= 2 * R; ++dz) {
       const int2 m = rowmeta(j, zb - R - 1 + dz);
 #pragma unroll
-      for (int s = 0; s < SX; ++s) q[r][s][dz] = meta_val(yv, m, lane + 64 * s);
+      for (int u = 0; u < NV; ++u)
+#pragma unroll
+        for (int s = 0; s < SX; ++s) q[u][r][s][dz] = meta_val(yv[u], m, lane + 64 * s);
     }
   }
   // prefetch everything plane z needs, for z = zb
@@ -154,14 +172,17 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
       const int2 mq = rowmeta(j, z + R);
       const int2 mo = rowmeta(j, z);
       mnext[r] = mo;
+      const int lo = mo.y & 0xffff, hi = mo.y >> 16;
 #pragma unroll
       for (int s = 0; s < SX; ++s) {
         const int i = lane + 64 * s;
-        nq[r][s] = meta_val(yv, mq, i);
-        const int lo = mo.y & 0xffff, hi = mo.y >> 16;
         const bool in = i >= lo && i < hi;
         nv[r][s] = in ? V[mo.x + i] : T(0);
-        nxv[r][s] = (in && xv) ? xv[mo.x + i] : T(0);
+#pragma unroll
+        for (int u = 0; u < NV; ++u) {
+          nq[u][r][s] = meta_val(yv[u], mq, i);
+          nxv[u][r][s] = (in && hasx) ? xv[u][mo.x + i] : T(0);
+        }
       }
     }
 #pragma unroll
@@ -171,7 +192,9 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
         const int tr = hr < R ? hr : hr + TY;
         const int2 mh = rowmeta(j0 - R + tr, z);
 #pragma unroll
-        for (int s = 0; s < SX; ++s) hn[t][s] = meta_val(yv, mh, lane + 64 * s);
+        for (int u = 0; u < NV; ++u)
+#pragma unroll
+          for (int s = 0; s < SX; ++s) hn[u][t][s] = meta_val(yv[u], mh, lane + 64 * s);
       }
     }
   };
@@ -179,40 +202,48 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
 
   for (int z = zb; z < ze; ++z) {
     const int b = (z - zb) & 1;
-    T* tb = tile[b];
     // commit the prefetched plane
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
       mcur[r] = mnext[r];
 #pragma unroll
       for (int s = 0; s < SX; ++s) {
-#pragma unroll
-        for (int d = 0; d < 2 * R; ++d) q[r][s][d] = q[r][s][d + 1];
-        q[r][s][2 * R] = nq[r][s];
         vq[r][s] = nv[r][s];
-        xq[r][s] = nxv[r][s];
+#pragma unroll
+        for (int u = 0; u < NV; ++u) {
+#pragma unroll
+          for (int d = 0; d < 2 * R; ++d) q[u][r][s][d] = q[u][r][s][d + 1];
+          q[u][r][s][2 * R] = nq[u][r][s];
+          xq[u][r][s] = nxv[u][r][s];
+        }
       }
     }
 #pragma unroll
-    for (int t = 0; t < HT; ++t)
+    for (int u = 0; u < NV; ++u)
 #pragma unroll
-      for (int s = 0; s < SX; ++s) hc[t][s] = hn[t][s];
+      for (int t = 0; t < HT; ++t)
+#pragma unroll
+        for (int s = 0; s < SX; ++s) hc[u][t][s] = hn[u][t][s];
     // issue the loads for the next plane
     if (z + 1 < ze) prefetch(z + 1);
     // plane z into LDS
 #pragma unroll
-    for (int r = 0; r < TYW; ++r) {
-      const int tr = wv + NW * r + R;
+    for (int u = 0; u < NV; ++u) {
+      T* tb = tile[b][u];
 #pragma unroll
-      for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = q[r][s][R];
-    }
+      for (int r = 0; r < TYW; ++r) {
+        const int tr = wv + NW * r + R;
 #pragma unroll
-    for (int t = 0; t < HT; ++t) {
-      const int hr = wv + NW * t;
-      if (hr < 2 * R) {
-        const int tr = hr < R ? hr : hr + TY;
+        for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = q[u][r][s][R];
+      }
 #pragma unroll
-        for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = hc[t][s];
+      for (int t = 0; t < HT; ++t) {
+        const int hr = wv + NW * t;
+        if (hr < 2 * R) {
+          const int tr = hr < R ? hr : hr + TY;
+#pragma unroll
+          for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = hc[u][t][s];
+        }
       }
     }
     lds_barrier();
@@ -226,18 +257,25 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
       for (int s = 0; s < SX; ++s) {
         const int i = lane + 64 * s;
         if (i >= lo && i < hi) {
-          const T yc = q[r][s][R];
-          T acc = T(g.w[0]) * yc;
-          const T* trow = tb + tr * LW + R + i;
+          const T va = ca * vq[r][s] + cb;
 #pragma unroll
-          for (int d = 1; d <= R; ++d) {
-            const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[r][s][R - d] + q[r][s][R + d];
-            acc += T(g.w[d]) * nb;
+          for (int u = 0; u < NV; ++u) {
+            const T yc = q[u][r][s][R];
+            T acc = T(g.w[0]) * yc;
+            const T* trow = tile[b][u] + tr * LW + R + i;
+#pragma unroll
+            for (int d = 1; d <= R; ++d) {
+              const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[u][r][s][R - d] +
+                           q[u][r][s][R + d];
+              acc += T(g.w[d]) * nb;
+            }
+            T o = acc + va * yc;
+            if (hasx) o += cc * xq[u][r][s];
+            if (act[u]) {
+              ov[u][mo.x + i] = o;
+              if (dot_part) dot += (double)o * (double)yc;
+            }
           }
-          T o = acc + (ca * vq[r][s] + cb) * yc;
-          if (xv) o += cc * xq[r][s];
-          ov[mo.x + i] = o;
-          if (dot_part) dot += (double)o * (double)yc;
         }
       }
     }
@@ -248,10 +286,29 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   }
 }
 
-template <typename T, int R, int SX>
-int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
-                double ca, double cb, double cc, double* dot_part, hipStream_t st) {
-  constexpr int TYW = (sizeof(T) == 4) ? (SX == 1 ? 8 : (SX == 2 ? 4 : 2)) : (SX == 1 ? 4 : (SX == 2 ? 2 : 1));
+// Tile shape per element type, x width (SX wavefront-widths) and vectors per
+// workgroup.  PC_ZTILE (environment, read once) selects among measured
+// alternatives for tuning runs; 0 is the default.
+int ztile_variant() {
+  static int v = [] {
+    const char* e = getenv("PC_ZTILE");
+    return e ? atoi(e) : 0;
+  }();
+  return v;
+}
+
+template <typename T, int R, int SX, int TYW, int NV>
+constexpr int zmarch_lds_bytes() {
+  return 2 * NV * (4 * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
+}
+
+template <typename T, int R, int SX, int TYW, int NV>
+int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+               double ca, double cb, double cc, double* dot_part, hipStream_t st) {
+  if constexpr (NV > 1 && zmarch_lds_bytes<T, R, SX, TYW, NV>() > 112 * 1024) {
+    // too much LDS for several vectors per workgroup: one vector per workgroup
+    return launch_cfg<T, R, SX, TYW, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  } else {
   constexpr int TY = 4 * TYW;
   StArgs a;
   a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
@@ -259,15 +316,34 @@ int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
   a.ld = ld;
   a.nslab = (gd->ny + TY - 1) / TY;
   a.nvec = nvec;
-  a.nblocks = a.nslab * nvec;
+  a.nblocks = a.nslab * ((nvec + NV - 1) / NV);
   a.nb8 = (a.nblocks + 7) / 8;
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
   for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
-  hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW>), dim3(8 * a.nb8), dim3(256), 0, st, a, Y, X, Out, V, T(ca), T(cb),
-                     T(cc), dot_part);
+  hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV>), dim3(8 * a.nb8), dim3(256), 0, st, a, Y, X, Out, V, T(ca),
+                     T(cb), T(cc), dot_part);
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
+  }
+}
+
+template <typename T, int R, int SX>
+int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+                double ca, double cb, double cc, double* dot_part, hipStream_t st) {
+  constexpr bool F = sizeof(T) == 4;
+  constexpr int TYW1 = F ? (SX == 1 ? 8 : (SX == 2 ? 4 : 2)) : (SX == 1 ? 4 : (SX == 2 ? 2 : 1));
+  constexpr int TYW2 = TYW1 > 1 ? TYW1 / 2 : 1;
+  constexpr int TYW4 = TYW2 > 1 ? TYW2 / 2 : 1;
+  constexpr int TYW8 = TYW4 > 1 ? TYW4 / 2 : 1;
+  // per-vector partial sums need one vector per workgroup
+  switch (dot_part ? 0 : ztile_variant()) {
+    case 1: return launch_cfg<T, R, SX, TYW2, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 2: return launch_cfg<T, R, SX, TYW1, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 3: return launch_cfg<T, R, SX, TYW4, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    case 4: return launch_cfg<T, R, SX, TYW8, 8>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    default: return launch_cfg<T, R, SX, TYW1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  }
 }
 
 template <typename T, int R>
@@ -412,6 +488,16 @@ int pc_internal_grid_view(void* grid, int* nx, int* ny, int* nz, const int** lo,
   *lo = gd->lo; *hi = gd->hi; *off = gd->off;
   return 0;
 }
+// Point map for single-vector kernels (the Hartree CG): row metadata
+// {off-lo, lo|hi<<16}, the row of every point, and the unscaled weights.
+int pc_internal_grid_points(void* grid, int* ny, int* nz, int* radius, const int2** meta, const int** prow,
+                            double* w8) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  *ny = gd->ny; *nz = gd->nz; *radius = gd->radius;
+  *meta = gd->meta; *prow = gd->prow;
+  for (int d = 0; d < 8; ++d) w8[d] = gd->w[d];
+  return 0;
+}
 long long pc_internal_grid_nrow(void* grid) {
   const GridDev* gd = static_cast<const GridDev*>(grid);
   return (long long)gd->ny * gd->nz;
@@ -468,6 +554,11 @@ int pc_grid_create(int nx, int ny, int nz, const int* row_lo, const int* row_hi,
   if (e == hipSuccess) e = hipMalloc(&g->off, nrow * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&g->zrange, 2 * ny * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&g->meta, nrow * sizeof(int2));
+  std::vector<int> prow((size_t)std::max(cnt, 1LL));
+  for (int r = 0; r < nrow; ++r)
+    for (int t = 0; t < hi[r] - lo[r]; ++t) prow[(size_t)off[r] + t] = r;
+  if (e == hipSuccess) e = hipMalloc(&g->prow, prow.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(g->prow, prow.data(), prow.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->meta, meta.data(), nrow * sizeof(int2), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->lo, lo.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->hi, hi.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
@@ -486,7 +577,7 @@ int pc_grid_destroy(void* grid) {
   GridDev* g = static_cast<GridDev*>(grid);
   if (!g) return pc::PC_OK;
   (void)hipFree(g->lo); (void)hipFree(g->hi); (void)hipFree(g->off); (void)hipFree(g->zrange);
-  (void)hipFree(g->meta);
+  (void)hipFree(g->meta); (void)hipFree(g->prow);
   delete g;
   return pc::PC_OK;
 }

@@ -1,111 +1,230 @@
 // Hartree solve: conjugate gradients on A x = rhs, A = -lap/h^2 on the grid,
 // FP64, no preconditioner (src/Eigensolvers/pcg.py:7-85; default cg_prec=0,
 // src/rsdft_models.py:31).  Stops when r.r <= tol^2 * r0.r0 or after maxit
-// iterations.  With alias_first=1 the second search direction is (1+beta) r_1,
+// iterations.  With alias_first=1 the second search direction is r_1 + beta r_1,
 // which is what the reference's CPU loop computes because its p and r are the
 // same array in the first iteration (pcg.py:45-77).
 //
-// All scalars stay on the device; each kernel checks the device "done" flag,
-// so the host only reads the flag back every `check_every` iterations.
+// Two launches per iteration, each a fixed grid of NB workgroups striding
+// over the domain points:
+//   cg_apply : p_new = r + beta p_old (formed on the fly at the centre and at
+//              every stencil neighbour), writes p_new and A p_new and one
+//              partial of p_new . A p_new per workgroup;
+//   cg_update: x += alpha p_new, r -= alpha A p_new, one partial of r.r per
+//              workgroup.
+// Every workgroup sums the NB partials it needs (r.r of the last two
+// iterations, p.Ap of this one) itself, in the same fixed order, so all
+// workgroups hold bit-identical alpha, beta and convergence decisions without
+// a separate reduction launch and without device-scope fences; workgroup 0
+// records the scalars for the host.  When the iteration has converged (or hit
+// maxit) both kernels return at once, so the host only reads the state back
+// every `check_every` iterations.
 #include "pc_common.h"
+#include <algorithm>
 
-int pc_internal_stencil_direct_f64(void* grid, const double* Y, const double* X, double* Out, const double* V,
-                                   double scale, double ca, double cb, double cc, double* dot_part, void* stream,
-                                   int* out_nblocks);
+int pc_internal_grid_points(void* grid, int* ny, int* nz, int* radius, const int2** meta, const int** prow,
+                            double* w8);
 long long pc_internal_grid_npts(void* grid);
-long long pc_internal_grid_nrow(void* grid);
 
 namespace {
 
-constexpr int kBlk = 256;
-enum { S_RO = 0, S_TOL1, S_DONE, S_ITS, S_ALPHA, S_BETA, S_MAXIT, S_COUNT = 8 };
+constexpr int kBlk = 512;       // threads per workgroup
+constexpr int kMaxNB = 512;     // workgroups per launch (= partials per reduction)
+// scalars for the host (written by workgroup 0); r.r of iteration t in slot S_RO0 + (t & 1)
+enum { S_RO0 = 0, S_RO1, S_TOL1, S_DONE, S_ITS, S_RES, S_COUNT = 8 };
+enum Mode { M_FIRST = 1, M_ALIAS = 2, M_NORMAL = 3 };
 
-__device__ double sum_parts(const double* part, int n, double* red) {
+struct PointMap {
+  const int2* meta;   // per row {off - lo, lo | hi << 16}
+  const int* prow;    // per point: its row j + ny*k
+  long long n;
+  long long chunk;    // points per workgroup (multiple of kBlk)
+  int ny, nz;
+  double w[8];        // weights of -lap/h^2; w[0] is the 3D centre weight
+};
+
+// Sum of part[0..nb) in a fixed order; the same in every workgroup.
+__device__ __forceinline__ double sum_parts(const double* part, int nb, double* red) {
   double s = 0.0;
-  for (int b = threadIdx.x; b < n; b += kBlk) s += part[b];
-  return pc_block_sum(s, red);
+  for (int b = threadIdx.x; b < nb; b += kBlk) s += part[b];
+  double t = pc_block_sum(s, red);
+  __shared__ double bc;
+  if (threadIdx.x == 0) bc = t;
+  __syncthreads();
+  return bc;
 }
 
-__global__ __launch_bounds__(kBlk) void cg_init_scalars(const double* part, int n, double tol, int maxit, double* sc) {
-  __shared__ double red[4];
-  const double ro = sum_parts(part, n, red);
-  if (threadIdx.x == 0) {
-    sc[S_RO] = ro;
-    sc[S_TOL1] = tol * tol * ro;
-    sc[S_ITS] = 0.0;
-    sc[S_MAXIT] = (double)maxit;
-    sc[S_DONE] = (maxit <= 0 || !(ro > tol * tol * ro)) ? 1.0 : 0.0;
+template <int MODE>
+__device__ __forceinline__ double qv(const double* __restrict__ a, const double* __restrict__ b, double beta,
+                                     long long idx) {
+  if (MODE == M_FIRST) return a[idx];
+  if (MODE == M_ALIAS) {
+    const double r = a[idx];
+    return r + beta * r;
   }
+  return a[idx] + beta * b[idx];
 }
 
-__global__ __launch_bounds__(kBlk) void cg_copy_rr(long long n, const double* __restrict__ r, double* __restrict__ p,
-                                                   double* __restrict__ part) {
-  __shared__ double red[4];
-  const long long i = (long long)blockIdx.x * kBlk + threadIdx.x;
+// sum_d w_d (6 neighbours of q at distance d) + w_0 q(p), q formed on the fly
+template <int R, int MODE>
+__device__ __forceinline__ double apply_point(const PointMap& g, const double* __restrict__ a,
+                                              const double* __restrict__ b, double beta, long long p, double& qc) {
+  const int row = g.prow[p];
+  const int2 m = g.meta[row];
+  const int i = (int)(p - m.x);
+  const int lo = m.y & 0xffff, hi = m.y >> 16;
+  const int k = row / g.ny;
+  const int j = row - k * g.ny;
+  qc = qv<MODE>(a, b, beta, p);
+  double acc = g.w[0] * qc;
+  auto qrow = [&](int2 mm) -> double {
+    const int l2 = mm.y & 0xffff, h2 = mm.y >> 16;
+    return (i >= l2 && i < h2) ? qv<MODE>(a, b, beta, (long long)mm.x + i) : 0.0;
+  };
+#pragma unroll
+  for (int d = 1; d <= R; ++d) {
+    double nb = ((i - d >= lo) ? qv<MODE>(a, b, beta, p - d) : 0.0) +
+                ((i + d < hi) ? qv<MODE>(a, b, beta, p + d) : 0.0);
+    if (j - d >= 0) nb += qrow(g.meta[row - d]);
+    if (j + d < g.ny) nb += qrow(g.meta[row + d]);
+    if (k - d >= 0) nb += qrow(g.meta[row - d * g.ny]);
+    if (k + d < g.nz) nb += qrow(g.meta[row + d * g.ny]);
+    acc += g.w[d] * nb;
+  }
+  return acc;
+}
+
+// r = rhs - A x; partial r.r -> part_rr
+template <int R>
+__global__ __launch_bounds__(kBlk) void cg_init(PointMap g, const double* __restrict__ x,
+                                                const double* __restrict__ rhs, double* __restrict__ r,
+                                                double* __restrict__ part_rr) {
+  __shared__ double red[kBlk / 64];
+  const long long p0 = (long long)blockIdx.x * g.chunk;
+  const long long p1 = min(p0 + g.chunk, g.n);
   double s = 0.0;
-  if (i < n) {
-    const double v = r[i];
-    p[i] = v;
-    s = v * v;
+  for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) {
+    double qc;
+    const double acc = apply_point<R, M_FIRST>(g, x, nullptr, 0.0, p, qc);
+    const double rv = rhs[p] - acc;
+    r[p] = rv;
+    s += rv * rv;
   }
   const double t = pc_block_sum(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
 }
 
-__global__ __launch_bounds__(kBlk) void cg_alpha(const double* part, int n, double* sc) {
-  __shared__ double red[4];
-  if (sc[S_DONE] != 0.0) return;
-  const double pap = sum_parts(part, n, red);
-  if (threadIdx.x == 0) sc[S_ALPHA] = sc[S_RO] / pap;
-}
-
-__global__ __launch_bounds__(kBlk) void cg_update_xr(long long n, const double* __restrict__ sc, double* __restrict__ x,
-                                                     double* __restrict__ r, const double* __restrict__ p,
-                                                     const double* __restrict__ ap, double* __restrict__ part) {
-  __shared__ double red[4];
-  if (sc[S_DONE] != 0.0) return;
-  const double alpha = sc[S_ALPHA];
-  const long long i = (long long)blockIdx.x * kBlk + threadIdx.x;
+// Iteration t (t completed so far).  rr_cur: partials of r_t.r_t; sc slot
+// S_RO0 + ((t-1)&1) holds r_{t-1}.r_{t-1} (t >= 1).  Decides convergence,
+// forms p_new = r + beta p_old, writes p_new, A p_new and partials of
+// p_new . A p_new.
+template <int R, int MODE>
+__global__ __launch_bounds__(kBlk) void cg_apply(PointMap g, int t, int maxit, double tol,
+                                                 const double* __restrict__ r, const double* __restrict__ p_old,
+                                                 double* __restrict__ p_new, double* __restrict__ ap,
+                                                 const double* __restrict__ part_rr, double* __restrict__ part_pq,
+                                                 double* sc) {
+  __shared__ double red[kBlk / 64];
+  if (t > 0 && sc[S_DONE] != 0.0) return;
+  const int nb = gridDim.x;
+  const double rr = sum_parts(part_rr, nb, red);
+  const double tol1 = t == 0 ? tol * tol * rr : sc[S_TOL1];
+  const bool done = t >= maxit || !(rr > tol1);
+  const double beta = t > 0 ? rr / sc[S_RO0 + ((t - 1) & 1)] : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    sc[S_RO0 + (t & 1)] = rr;
+    if (t == 0) sc[S_TOL1] = tol1;
+    sc[S_ITS] = (double)t;
+    sc[S_RES] = rr;
+    sc[S_DONE] = done ? 1.0 : 0.0;
+  }
+  if (done) return;
+  const long long p0 = (long long)blockIdx.x * g.chunk;
+  const long long p1 = min(p0 + g.chunk, g.n);
   double s = 0.0;
-  if (i < n) {
+  for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) {
+    double qc;
+    const double acc = apply_point<R, MODE>(g, r, p_old, beta, p, qc);
+    p_new[p] = qc;
+    ap[p] = acc;
+    s += qc * acc;
+  }
+  const double tt = pc_block_sum(s, red);
+  if (threadIdx.x == 0) part_pq[blockIdx.x] = tt;
+}
+
+// alpha = r_t.r_t / p.Ap; x += alpha p; r -= alpha Ap; partials of r_{t+1}.r_{t+1}
+__global__ __launch_bounds__(kBlk) void cg_update(long long n, long long chunk, int t, const double* sc,
+                                                  double* __restrict__ x, double* __restrict__ r,
+                                                  const double* __restrict__ p, const double* __restrict__ ap,
+                                                  const double* __restrict__ part_pq,
+                                                  double* __restrict__ part_rr_next) {
+  __shared__ double red[kBlk / 64];
+  if (sc[S_DONE] != 0.0) return;
+  const double pq = sum_parts(part_pq, gridDim.x, red);
+  const double alpha = sc[S_RO0 + (t & 1)] / pq;
+  const long long p0 = (long long)blockIdx.x * chunk;
+  const long long p1 = min(p0 + chunk, n);
+  double s = 0.0;
+  for (long long i = p0 + threadIdx.x; i < p1; i += kBlk) {
     x[i] += alpha * p[i];
     const double rv = r[i] - alpha * ap[i];
     r[i] = rv;
-    s = rv * rv;
+    s += rv * rv;
   }
-  const double t = pc_block_sum(s, red);
-  if (threadIdx.x == 0) part[blockIdx.x] = t;
+  const double tt = pc_block_sum(s, red);
+  if (threadIdx.x == 0) part_rr_next[blockIdx.x] = tt;
 }
 
-__global__ __launch_bounds__(kBlk) void cg_beta(const double* part, int n, double* sc) {
-  __shared__ double red[4];
-  if (sc[S_DONE] != 0.0) return;
-  const double ro1 = sum_parts(part, n, red);
-  if (threadIdx.x == 0) {
-    const double its = sc[S_ITS] + 1.0;
-    sc[S_BETA] = ro1 / sc[S_RO];
-    sc[S_RO] = ro1;
-    sc[S_ITS] = its;
-    // the next p update still runs for this iteration; done takes effect after it
-    sc[S_DONE] = (its >= sc[S_MAXIT] || !(ro1 > sc[S_TOL1])) ? 2.0 : 0.0;
-  }
+struct Work {
+  double* r;
+  double* p[2];
+  double* ap;
+  double* prr[2];
+  double* ppq;
+  double* sc;
+};
+
+size_t work_bytes_for(long long n) {
+  return (size_t)(4 * n + 3 * kMaxNB + S_COUNT) * sizeof(double);
 }
 
-__global__ __launch_bounds__(kBlk) void cg_update_p(long long n, double* __restrict__ sc, const double* __restrict__ r,
-                                                    double* __restrict__ p, int alias_first) {
-  const double done = sc[S_DONE];
-  if (done == 1.0) return;
-  const double beta = sc[S_BETA];
-  const bool first = alias_first && sc[S_ITS] == 1.0;
-  const long long i = (long long)blockIdx.x * kBlk + threadIdx.x;
-  if (i < n) {
-    const double rv = r[i];
-    p[i] = rv + beta * (first ? rv : p[i]);
-  }
+template <int R>
+int launch_init(int nb, hipStream_t st, const PointMap& g, const double* x, const double* rhs, const Work& w) {
+  hipLaunchKernelGGL((cg_init<R>), dim3(nb), dim3(kBlk), 0, st, g, x, rhs, w.r, w.prr[0]);
+  return hipGetLastError() == hipSuccess ? pc::PC_OK : pc::PC_ERR_HIP;
 }
 
-__global__ void cg_finish_iter(double* sc) {
-  if (threadIdx.x == 0 && sc[S_DONE] == 2.0) sc[S_DONE] = 1.0;
+template <int R>
+int launch_apply(int mode, int nb, hipStream_t st, const PointMap& g, int t, int maxit, double tol, const Work& w) {
+  const double* p_old = w.p[t & 1];
+  double* p_new = w.p[(t + 1) & 1];
+  const double* prr = w.prr[t & 1];
+  switch (mode) {
+    case M_FIRST:
+      hipLaunchKernelGGL((cg_apply<R, M_FIRST>), dim3(nb), dim3(kBlk), 0, st, g, t, maxit, tol, w.r, p_old, p_new,
+                         w.ap, prr, w.ppq, w.sc);
+      break;
+    case M_ALIAS:
+      hipLaunchKernelGGL((cg_apply<R, M_ALIAS>), dim3(nb), dim3(kBlk), 0, st, g, t, maxit, tol, w.r, p_old, p_new,
+                         w.ap, prr, w.ppq, w.sc);
+      break;
+    default:
+      hipLaunchKernelGGL((cg_apply<R, M_NORMAL>), dim3(nb), dim3(kBlk), 0, st, g, t, maxit, tol, w.r, p_old, p_new,
+                         w.ap, prr, w.ppq, w.sc);
+      break;
+  }
+  return hipGetLastError() == hipSuccess ? pc::PC_OK : pc::PC_ERR_HIP;
+}
+
+struct Launchers {
+  int (*init)(int, hipStream_t, const PointMap&, const double*, const double*, const Work&);
+  int (*apply)(int, int, hipStream_t, const PointMap&, int, int, double, const Work&);
+};
+
+template <int R>
+Launchers launchers() {
+  return {launch_init<R>, launch_apply<R>};
 }
 
 }  // namespace
@@ -114,10 +233,7 @@ extern "C" {
 
 int pc_pcg_workspace(void* grid, size_t* out_bytes) {
   PC_CHECK_ARG(grid && out_bytes, "null pointer");
-  const long long n = pc_internal_grid_npts(grid);
-  const long long nb_st = (pc_internal_grid_nrow(grid) + 3) / 4;
-  const long long nb_vec = (n + kBlk - 1) / kBlk;
-  *out_bytes = (size_t)(3 * n + nb_st + nb_vec + S_COUNT) * sizeof(double);
+  *out_bytes = work_bytes_for(pc_internal_grid_npts(grid));
   return pc::PC_OK;
 }
 
@@ -127,50 +243,72 @@ int pc_pcg_workspace(void* grid, size_t* out_bytes) {
 int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
                  int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream) {
   PC_CHECK_ARG(grid && rhs && x && work && its_out, "null pointer");
-  size_t need = 0;
-  pc_pcg_workspace(grid, &need);
-  PC_CHECK_ARG(work_bytes >= need, "workspace too small");
+  const long long n = pc_internal_grid_npts(grid);
+  PC_CHECK_ARG(n > 0, "empty domain");
+  PC_CHECK_ARG(work_bytes >= work_bytes_for(n), "workspace too small");
+  PC_CHECK_ARG(((uintptr_t)work & 7) == 0, "workspace not 8-byte aligned");
+  PC_CHECK_ARG(maxit >= 0, "maxit < 0");
   if (check_every < 1) check_every = 8;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const long long n = pc_internal_grid_npts(grid);
-  const int nb_vec = (int)((n + kBlk - 1) / kBlk);
-  double* r = static_cast<double*>(work);
-  double* p = r + n;
-  double* ap = p + n;
-  const long long nb_st_max = (pc_internal_grid_nrow(grid) + 3) / 4;
-  double* part_a = ap + n;
-  double* part_b = part_a + nb_st_max;
-  double* sc = part_b + nb_vec;
-  int nb_st = 0;
-  // r = rhs - A x
-  int s = pc_internal_stencil_direct_f64(grid, x, rhs, r, x, -inv_h2, 0.0, 0.0, 1.0, nullptr, stream, &nb_st);
-  if (s) return s;
-  PC_CHECK_ARG(nb_st <= nb_st_max, "partials overflow");
-  hipLaunchKernelGGL(cg_copy_rr, dim3(nb_vec), dim3(kBlk), 0, st, n, r, p, part_b);
-  PC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(cg_init_scalars, dim3(1), dim3(kBlk), 0, st, part_b, nb_vec, tol, maxit, sc);
-  PC_LAUNCH_CHECK();
+
+  PointMap g;
+  int radius = 0;
+  double w8[8];
+  pc_internal_grid_points(grid, &g.ny, &g.nz, &radius, &g.meta, &g.prow, w8);
+  g.n = n;
+  for (int d = 0; d < 8; ++d) g.w[d] = w8[d] * inv_h2;
+  g.w[0] *= 3.0;
+  Launchers L;
+  switch (radius) {
+    case 1: L = launchers<1>(); break;
+    case 2: L = launchers<2>(); break;
+    case 3: L = launchers<3>(); break;
+    case 4: L = launchers<4>(); break;
+    case 6: L = launchers<6>(); break;
+    default: PC_CHECK_ARG(false, "unsupported stencil radius");
+  }
+  // workgroups own contiguous chunks (multiples of kBlk points)
+  const long long blocks_of_pts = (n + kBlk - 1) / kBlk;
+  const int nb = (int)std::min<long long>(kMaxNB, blocks_of_pts);
+  g.chunk = ((blocks_of_pts + nb - 1) / nb) * kBlk;
+  const int nb_used = (int)((n + g.chunk - 1) / g.chunk);
+
+  Work w;
+  w.r = static_cast<double*>(work);
+  w.p[0] = w.r + n;
+  w.p[1] = w.p[0] + n;
+  w.ap = w.p[1] + n;
+  w.prr[0] = w.ap + n;
+  w.prr[1] = w.prr[0] + kMaxNB;
+  w.ppq = w.prr[1] + kMaxNB;
+  w.sc = w.ppq + kMaxNB;
+
+  if (L.init(nb_used, st, g, x, rhs, w)) {
+    pc::set_error("pc_pcg_solve: launch failed");
+    return pc::PC_ERR_HIP;
+  }
   // pinned readback buffer, allocated once per host thread and kept
   static thread_local double* host = nullptr;
-  if (!host) PC_HIP(hipHostMalloc(&host, 4 * sizeof(double), hipHostMallocDefault));
+  if (!host) PC_HIP(hipHostMalloc(&host, S_COUNT * sizeof(double), hipHostMallocDefault));
   int status = pc::PC_OK;
-  for (int it0 = 0; it0 < maxit; it0 += check_every) {
-    for (int t = 0; t < check_every && it0 + t < maxit; ++t) {
-      status = pc_internal_stencil_direct_f64(grid, p, nullptr, ap, x, inv_h2, 0.0, 0.0, 0.0, part_a, stream, &nb_st);
-      if (status) break;
-      hipLaunchKernelGGL(cg_alpha, dim3(1), dim3(kBlk), 0, st, part_a, nb_st, sc);
-      hipLaunchKernelGGL(cg_update_xr, dim3(nb_vec), dim3(kBlk), 0, st, n, sc, x, r, p, ap, part_b);
-      hipLaunchKernelGGL(cg_beta, dim3(1), dim3(kBlk), 0, st, part_b, nb_vec, sc);
-      hipLaunchKernelGGL(cg_update_p, dim3(nb_vec), dim3(kBlk), 0, st, n, sc, r, p, alias_first);
-      hipLaunchKernelGGL(cg_finish_iter, dim3(1), dim3(64), 0, st, sc);
-      if (hipGetLastError() != hipSuccess) {
+  // iteration t: apply(t) decides convergence from r_t; update(t) makes r_{t+1}.
+  // apply(maxit) only records the final state.
+  for (int t0 = 0; t0 <= maxit; t0 += check_every) {
+    for (int t = t0; t < t0 + check_every && t <= maxit; ++t) {
+      const int mode = t == 0 ? M_FIRST : ((t == 1 && alias_first) ? M_ALIAS : M_NORMAL);
+      status = L.apply(mode, nb_used, st, g, t, maxit, tol, w);
+      if (!status && t < maxit) {
+        hipLaunchKernelGGL(cg_update, dim3(nb_used), dim3(kBlk), 0, st, n, g.chunk, t, w.sc, x, w.r,
+                           w.p[(t + 1) & 1], w.ap, w.ppq, w.prr[(t + 1) & 1]);
+        if (hipGetLastError() != hipSuccess) status = pc::PC_ERR_HIP;
+      }
+      if (status) {
         pc::set_error("pc_pcg_solve: kernel launch failed");
-        status = pc::PC_ERR_HIP;
         break;
       }
     }
     if (status) break;
-    if (hipMemcpyAsync(host, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+    if (hipMemcpyAsync(host, w.sc, S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
       pc::set_error("pc_pcg_solve: readback failed");
       status = pc::PC_ERR_HIP;
@@ -179,16 +317,14 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
     if (host[S_DONE] != 0.0) break;
   }
   if (!status) {
-    if (hipMemcpyAsync(host, sc, 4 * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
-        hipStreamSynchronize(st) != hipSuccess) {
-      pc::set_error("pc_pcg_solve: readback failed");
-      status = pc::PC_ERR_HIP;
-    } else {
-      *its_out = (int)host[S_ITS];
-      if (res_out) {
-        res_out[0] = host[S_RO];
-        res_out[1] = host[S_TOL1];
-      }
+    if (host[S_DONE] == 0.0) {
+      pc::set_error("pc_pcg_solve: internal error (no final state)");
+      return pc::PC_ERR_HIP;
+    }
+    *its_out = (int)host[S_ITS];
+    if (res_out) {
+      res_out[0] = host[S_RES];
+      res_out[1] = host[S_TOL1];
     }
   }
   return status;

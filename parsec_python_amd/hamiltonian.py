@@ -47,7 +47,7 @@ class Hamiltonian:
         self.pot32 = None
         self.set_potential(pot)
         # optional per-launch timing of the multi-vector stencil: list of
-        # (start event, end event, algorithmic bytes)
+        # (start event, end event, algorithmic bytes, element size)
         self.stencil_timer: list | None = None
 
     # ------------------------------------------------------------------ potential
@@ -98,7 +98,7 @@ class Hamiltonian:
         if timed:
             e1.record()
             esz = 8 if dt == F64 else 4
-            self.stencil_timer.append((e0, e1, (3 if X is not None else 2) * self.n * nvec * esz))
+            self.stencil_timer.append((e0, e1, (3 if X is not None else 2) * self.n * nvec * esz, esz))
         if cbuf is not None:
             _lib.call(f"pc_proj_scatter_{suffix}", self.proj.handle, ptr(cbuf), ptr(out), ld, nvec, a, st)
 

@@ -160,6 +160,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     lam = None
     occ = None
     hist = []
+    first_filt_info = {}
     e_prev = None
     energies = None
     converged = False
@@ -170,6 +171,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         H.set_potential(pot)
         if its == 1:
             W, lam = eig.first_filt(k, cfg.poldeg)
+            first_filt_info = dict(eig.last_first_filt)
         else:
             W, lam = eig.chefsi1(W, lam, cfg.poldeg, nev)
         t1 = time.perf_counter()
@@ -239,6 +241,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     res = SCFResult(energies=energies, eigenvalues=np.asarray(lam), occupations=np.asarray(occ), iterations=its,
                     error=err, converged=converged, history=hist, timings=tim, rho=rho, hpot=hpot, pot=pot, W=W,
                     grid=prob.grid, setup_info={"setup_time": prob.setup_time, **prob.diag_info,
-                                                "filter_steps": eig.stats["filter_steps"], "rr": eig.stats["rr"]})
+                                                "filter_steps": eig.stats["filter_steps"], "rr": eig.stats["rr"],
+                                                "first_filt": first_filt_info})
     eig.close()
     return res
