@@ -1,11 +1,15 @@
 """Benchmark: SCF time-to-converge on one MI355X.
 
-A step is one full SCF solve of the workload from the SAD start, stopped at
-|E_i - E_{i-1}| < 1e-6 Ha/atom; the operator, SAD density and projector tables
-are built (and resident in HBM) before the timed region.  The solve is run in
-the mixed-precision configuration (FP32 filter / H*X / subspace GEMMs, FP64
-Cholesky, eigensolver, Hartree CG, mixer and energies, finishing in FP64); the
-plain FP64 solve of the same problem is timed beside it.
+A step is one full SCF solve of the workload, stopped at |E_i - E_{i-1}| <
+1e-6 Ha/atom; the operator, SAD density and projector tables are built (and
+resident in HBM) before the timed region.  The timed solve is the mixed-
+precision configuration (FP32 filter / H*X / subspace GEMMs with FP64-
+accumulated Gram matrices; FP64 Cholesky, eigensolver, Hartree CG, mixer and
+energies) started from the learned density (the network's forward pass and the
+Hartree correction of its start density are inside the timed region; the
+weights are loaded before).  The plain FP64 solve from the SAD start is timed
+beside it, as is the mixed solve from the SAD start.  Accuracy is reported
+against a tightly converged FP64 solve (|dE| < 1e-9 Ha/atom, untimed).
 
 Prints ONE JSON line (rank 0, N=1).  ``roofline`` is for the dominant kernel,
 the multi-vector fused stencil (HBM-bound): achieved = algorithmic bytes
@@ -73,13 +77,20 @@ def measured_traffic(name):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
-def solve(prob, timer=None):
+def solve(prob, timer=None, net=None):
+    """One timed solve; with ``net`` the learned start (network forward pass and
+    its Hartree correction) is inside the timed region."""
     import torch
     from parsec_python_amd.scf import run_scf
     prob.H.stencil_timer = timer
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    res = run_scf(prob)
+    if net is None:
+        res = run_scf(prob)
+    else:
+        from parsec_python_amd.learned import learned_start
+        rho1, vh1 = learned_start(prob, net)
+        res = run_scf(prob, rho_init=rho1, hpot_init=vh1)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prob.H.stencil_timer = None
@@ -143,7 +154,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="si87h76", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--learned", default=None, help="path of a trained initialiser checkpoint (.npz)")
+    ap.add_argument("--learned", default="default",
+                    help="trained initialiser checkpoint (.npz); 'default' = the packaged one if present, 'none' = SAD")
+    ap.add_argument("--no-reference", action="store_true", help="skip the tightly converged FP64 reference energy")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -165,13 +178,34 @@ def main():
     del prob64
     torch.cuda.empty_cache()
 
+    e_ref = None
+    if not args.no_reference:
+        probr = SCFProblem(atoms, dom, nev, SCFConfig(precision="fp64", **dict(common, etol_ha_per_atom=1e-9,
+                                                                                maxits=40)))
+        resr, tr = solve(probr)
+        e_ref = 0.5 * resr.energies["total_ry"] / probr.n_atoms
+        _log(f"fp64 reference (|dE| < 1e-9 Ha/atom or 40 iterations): {resr.iterations} iterations, {tr:.3f} s")
+        probr.close()
+        del probr
+        torch.cuda.empty_cache()
+
+    net = None
+    ckpt = args.learned
+    if ckpt == "default":
+        from parsec_python_amd.learned import default_checkpoint
+        ckpt = default_checkpoint() if os.path.exists(default_checkpoint()) else "none"
+    if ckpt != "none":
+        from parsec_python_amd.learned import UNet
+        net = UNet.load(ckpt)
     prob = SCFProblem(atoms, dom, nev, SCFConfig(precision="mixed", **common))
+    res_sad, t_sad = solve(prob)
+    _log(f"mixed from SAD: {res_sad.iterations} iterations, {t_sad:.3f} s")
     for _ in range(args.warmup):
-        r, t = solve(prob)
+        r, t = solve(prob, net=net)
         _log(f"mixed warmup: {r.iterations} iterations, {t:.3f} s")
     times, results, timer = [], [], []
     for _ in range(args.steps):
-        res, t = solve(prob, timer)
+        res, t = solve(prob, timer, net=net)
         _log(f"mixed solve: {res.iterations} iterations, {t:.3f} s")
         times.append(t)
         results.append(res)
@@ -200,17 +234,21 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "fp32+fp64",
+        "dtype": "fp32+fp64" + ("+bf16" if net is not None else ""),
         "data": "synthetic geometry (built-in coordinates), SAD start",
         "config": {"workload": WORKLOADS[args.workload]["desc"], "domain": "sphere", "points": prob.n,
-                   "block_k": nev + 15, "learned_init": False,
+                   "block_k": nev + 15, "learned_init": net is not None,
+                   "learned_checkpoint": os.path.basename(ckpt) if net is not None else None,
                    "stop": "|dE| < 1e-6 Ha/atom between SCF iterations"},
         "fp64_time_to_converge_s": round(t64, 4),
         "speedup_vs_fp64": round(t64 / value, 3),
-        "iterations": {"mixed": res.iterations, "fp64": res64.iterations},
+        "iterations": {"mixed": res.iterations, "fp64": res64.iterations, "mixed_sad_start": res_sad.iterations},
+        "mixed_sad_start_time_s": round(t_sad, 4),
         "ms_per_iteration": {"mixed": round(1e3 * value / res.iterations, 3),
                              "fp64": round(1e3 * t64 / res64.iterations, 3)},
-        "error_vs_fp64": {"energy_ha_per_atom": abs(e_mixed - e_fp64), "max_occupied_eig_ha": dlam},
+        "error_vs_fp64": {"energy_ha_per_atom": abs(e_mixed - e_fp64), "max_occupied_eig_ha": dlam,
+                          "mixed_vs_converged_fp64_ha_per_atom": None if e_ref is None else abs(e_mixed - e_ref),
+                          "fp64_stop_vs_converged_fp64_ha_per_atom": None if e_ref is None else abs(e_fp64 - e_ref)},
         "energy_ry": {"mixed": res.energies["total_ry"], "fp64": res64.energies["total_ry"]},
         "roofline": {"kernel": "stencil_zmarch<float> (fused FD stencil + V + Chebyshev step)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

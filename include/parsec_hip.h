@@ -132,8 +132,9 @@ int pc_rand_f32(long long n, int ncol, long long ld, unsigned long long seed, in
  * replaces: src/Eigensolvers/pcg.py:7-85 (CPU) / pcg_gpu.py:22-65 (CuPy).
  * Solves (-lap/h^2) x = rhs, FP64, warm start x, stop when r.r <= tol^2 r0.r0
  * or maxit; alias_first=1 reproduces the CPU reference's first-iteration p/r
- * aliasing.  Synchronises every check_every iterations.  res_out (host, may be
- * NULL) receives {final r.r, tol^2 * r0.r0}. */
+ * aliasing.  Two launches per iteration (stencil + direction update fused;
+ * x/r update), no separate reductions.  Synchronises every check_every
+ * iterations.  res_out (host, may be NULL) receives {final r.r, tol^2 r0.r0}. */
 int pc_pcg_workspace(void* grid, size_t* out_bytes);
 int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
                  int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream);
@@ -146,6 +147,44 @@ int pc_sad_eval(void* grid, double h, double rad, int natoms, const double* pos,
                 const double* d, double* pot, double* rho, double* hpot0, void* stream);
 int pc_spline_eval(const double* xi, const double* z, const double* c, const double* d, int n, const double* r,
                    long long m, double* out, void* stream);
+
+/* ---- learned density initialiser: 3D U-Net kernels -----------------------------
+ * The reference has no network code: its ML start reads a density file
+ * (src/V_ion/pseudoDiag_ML4Den_poisson.py:392-540).  These kernels train and
+ * run the network that produces that density.  Tensors are BF16 (raw 16-bit
+ * patterns) or FP32, channel-last: (voxel v, channel c) at v*ld + c,
+ * v = (z*H + y)*W + x.  Weights W[co][t][ci], t = (dz+1)*9 + (dy+1)*3 + (dx+1).
+ * conv3d_fwd: 3x3x3, zero padding, bf16 MFMA with FP32 accumulation;
+ *   cin = 1 or a multiple of 16; mode 0 = FP32 out + bias, 1 = leaky ReLU ->
+ *   BF16 out, 2 = FP32 out without bias.
+ * conv3d_wflip: weights of the input-gradient convolution (taps reversed,
+ *   channels transposed).
+ * conv3d_wgrad: dW[co][t][ci] = sum_v dZ[v][co] X[v + t][ci] (FP32).
+ * act_bwd: dZ = dA * leaky'(a) (a NULL: identity) rounded to BF16, and the
+ *   bias gradient sum_v dZ.
+ * pool2 / up2: 2x2x2 average pooling and nearest upsampling, with backward.
+ * adam: bias-corrected Adam on FP32 master weights, writes the BF16 copy.
+ * mse: dy = 2 (y - t) / n and sum (y - t)^2 (FP64, device). */
+int pc_unet_conv3d_fwd(const void* X, int ldx, int cin, const void* W, const float* bias, int cout, int D, int H,
+                       int Wd, int mode, float slope, void* Y, int ldy, void* stream);
+int pc_unet_conv3d_wflip(const void* W, int cout, int cin, void* Wt, void* stream);
+int pc_unet_wgrad_workspace(int cout, int cin, size_t* out_bytes);
+int pc_unet_conv3d_wgrad(const void* dZ, int ldz, int cout, const void* X, int ldx, int cin, int D, int H, int Wd,
+                         float* dW, void* work, size_t work_bytes, void* stream);
+int pc_unet_act_bwd_workspace(int C, long long V, size_t* out_bytes);
+int pc_unet_act_bwd(const float* dA, int lda, const void* a, int lda_a, int C, int D, int H, int Wd, float slope,
+                    void* dZ, int ldz, float* db, void* work, size_t work_bytes, void* stream);
+int pc_unet_pool2_fwd(const void* X, int ldx, int C, int D, int H, int Wd, void* Y, int ldy, void* stream);
+int pc_unet_pool2_bwd(const float* dY, int ldy, const float* dS, int lds, int C, int D, int H, int Wd, float* dX,
+                      int ldx, void* stream);
+int pc_unet_up2_fwd(const void* X, int ldx, int C, int D, int H, int Wd, void* Y, int ldy, void* stream);
+int pc_unet_up2_bwd(const float* dY, int ldy, int C, int D, int H, int Wd, float* dX, int ldx, void* stream);
+int pc_unet_cast_bf16(const float* src, long long n, void* dst, void* stream);
+int pc_unet_adam(float* w, const float* g, float* m, float* v, long long n, double lr, double b1, double b2,
+                 double eps, double wd, int step, void* wbf, void* stream);
+int pc_unet_mse_workspace(size_t* out_bytes);
+int pc_unet_mse(const float* y, const float* t, long long n, float* dy, double* loss_dev, void* work,
+                size_t work_bytes, void* stream);
 
 #ifdef __cplusplus
 }

@@ -24,7 +24,6 @@
 #include "pc_common.h"
 #include <vector>
 #include <algorithm>
-#include <cstdlib>
 
 namespace {
 
@@ -286,17 +285,6 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   }
 }
 
-// Tile shape per element type, x width (SX wavefront-widths) and vectors per
-// workgroup.  PC_ZTILE (environment, read once) selects among measured
-// alternatives for tuning runs; 0 is the default.
-int ztile_variant() {
-  static int v = [] {
-    const char* e = getenv("PC_ZTILE");
-    return e ? atoi(e) : 0;
-  }();
-  return v;
-}
-
 template <typename T, int R, int SX, int TYW, int NV>
 constexpr int zmarch_lds_bytes() {
   return 2 * NV * (4 * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
@@ -328,22 +316,22 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
   }
 }
 
+// One y-row per wavefront (TYW = 1) and as many vectors per workgroup as keep
+// the two LDS tiles within 56 KB (two workgroups per CU): 4 FP32 vectors at
+// radius 4, 2 FP64 vectors, 2 FP32 vectors at radius 6.  Measured on the
+// Si87H76 and 128^3 workloads against taller slabs and 1..8 vectors.
 template <typename T, int R, int SX>
 int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                 double ca, double cb, double cc, double* dot_part, hipStream_t st) {
-  constexpr bool F = sizeof(T) == 4;
-  constexpr int TYW1 = F ? (SX == 1 ? 8 : (SX == 2 ? 4 : 2)) : (SX == 1 ? 4 : (SX == 2 ? 2 : 1));
-  constexpr int TYW2 = TYW1 > 1 ? TYW1 / 2 : 1;
-  constexpr int TYW4 = TYW2 > 1 ? TYW2 / 2 : 1;
-  constexpr int TYW8 = TYW4 > 1 ? TYW4 / 2 : 1;
-  // per-vector partial sums need one vector per workgroup
-  switch (dot_part ? 0 : ztile_variant()) {
-    case 1: return launch_cfg<T, R, SX, TYW2, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-    case 2: return launch_cfg<T, R, SX, TYW1, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-    case 3: return launch_cfg<T, R, SX, TYW4, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-    case 4: return launch_cfg<T, R, SX, TYW8, 8>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-    default: return launch_cfg<T, R, SX, TYW1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-  }
+  constexpr int kLds = 56 * 1024;
+  if (dot_part)   // per-vector partial sums need one vector per workgroup
+    return launch_cfg<T, R, SX, 1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  if constexpr (zmarch_lds_bytes<T, R, SX, 1, 4>() <= kLds)
+    return launch_cfg<T, R, SX, 1, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  else if constexpr (zmarch_lds_bytes<T, R, SX, 1, 2>() <= kLds)
+    return launch_cfg<T, R, SX, 1, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  else
+    return launch_cfg<T, R, SX, 1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
 }
 
 template <typename T, int R>
@@ -597,11 +585,9 @@ int pc_stencil_f32(void* grid, const float* Y, const float* X, float* Out, const
 int pc_stencil_num_partials(void* grid, int dtype_bytes, int nvec, int* out_count, int* out_per_vector) {
   const GridDev* gd = static_cast<const GridDev*>(grid);
   PC_CHECK_ARG(gd && out_count && out_per_vector, "null pointer");
-  const int sx = gd->nx <= 64 ? 1 : (gd->nx <= 128 ? 2 : 4);
-  int tyw;
-  if (dtype_bytes == 4) tyw = sx == 1 ? 8 : (sx == 2 ? 4 : 2);
-  else tyw = sx == 1 ? 4 : (sx == 2 ? 2 : 1);
-  const int nslab = (gd->ny + 4 * tyw - 1) / (4 * tyw);
+  PC_CHECK_ARG(dtype_bytes == 4 || dtype_bytes == 8, "dtype_bytes must be 4 or 8");
+  // with partial sums: one vector and one y-row per wavefront, 4 rows per workgroup
+  const int nslab = (gd->ny + 3) / 4;
   *out_per_vector = nslab;
   *out_count = nslab * nvec;
   return pc::PC_OK;

@@ -69,6 +69,44 @@ def test_stencil_step_vs_oracle(gpu, nx, ny, nz, kind, radius, dtype, nvec):
     H.close()
 
 
+@pytest.mark.parametrize("entry", ["pc_stencil_f64", "pc_stencil_direct_f64"])
+def test_stencil_dot_partials(gpu, entry):
+    """Per-workgroup partial sums of out*y (vector-major) add up to the oracle's dots."""
+    import ctypes as C
+    import torch
+    from parsec_python_amd import _lib
+    from parsec_python_amd.device import ptr, stream
+    from parsec_python_amd.grid import DeviceGrid
+    from parsec_python_amd.hamiltonian import leading_dim
+    g = _grid(21, 21, 21, 0.45, "sphere")
+    n, nvec, radius = g.npts, 9, 4
+    print(f"dot partials {entry}", flush=True)
+    rng = np.random.default_rng(3)
+    V = rng.standard_normal(n)
+    Y = rng.standard_normal((nvec, n))
+    ld = leading_dim(n)
+    dg = DeviceGrid(g, radius)
+    Yd = torch.zeros((nvec, ld), dtype=torch.float64, device="cuda")
+    Yd[:, :n] = torch.tensor(Y)
+    Od = torch.zeros_like(Yd)
+    Vd = torch.tensor(V, device="cuda")
+    if entry == "pc_stencil_f64":
+        cnt, per = C.c_int(), C.c_int()
+        _lib.call("pc_stencil_num_partials", dg.handle, 8, nvec, C.byref(cnt), C.byref(per))
+        nper = per.value
+    else:
+        nper = (g.ny * g.nz + 3) // 4
+    part = torch.zeros(nvec * nper, dtype=torch.float64, device="cuda")
+    _lib.call(entry, dg.handle, ptr(Yd), C.c_void_p(0), ptr(Od), ptr(Vd), nvec, ld, 0.5 / 0.45 ** 2, 1.0, 0.0, 0.0,
+              ptr(part), stream())
+    torch.cuda.synchronize()
+    L = _oracle_lap(g, radius) * (0.5 / 0.45 ** 2)
+    ref_out = (L @ Y.T).T + V * Y
+    dots = part.view(nvec, nper).sum(dim=1).cpu().numpy()
+    assert np.allclose(dots, np.sum(ref_out * Y, axis=1), rtol=1e-11)
+    dg.close()
+
+
 def test_projectors_vs_oracle(gpu):
     import torch
     from parsec_python_amd import ionic, molecules as M

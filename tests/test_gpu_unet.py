@@ -77,8 +77,9 @@ def test_conv3d_wgrad(gpu, cin, cout):
     _lib.call("pc_unet_wgrad_workspace", cout, cin, C.byref(nb))
     work = torch.empty(nb.value, dtype=torch.uint8, device="cuda")
     dW = torch.empty(cout * 27 * cin, dtype=torch.float32, device="cuda")
-    _lib.call("pc_unet_conv3d_wgrad", ptr(_bf16_tensor(dz)), cout, cout, ptr(_bf16_tensor(x)), cin, cin, D, H, W,
-              ptr(dW), ptr(work), nb.value, stream())
+    dZt, Xt = _bf16_tensor(dz), _bf16_tensor(x)     # keep both alive until the kernel has run
+    _lib.call("pc_unet_conv3d_wgrad", ptr(dZt), cout, cout, ptr(Xt), cin, cin, D, H, W, ptr(dW), ptr(work),
+              nb.value, stream())
     torch.cuda.synchronize()
     _, ref = U.conv3d_bwd(x.T.reshape(cin, D, H, W), np.zeros((cout, cin, 27)), dz.T.reshape(cout, D, H, W))
     got = dW.cpu().numpy().reshape(cout, 27, cin).transpose(0, 2, 1)
