@@ -78,7 +78,7 @@ __device__ __forceinline__ T meta_val(const T* __restrict__ y, int2 m, int i) {
 // V and x) are already in flight.  A workgroup handles NV vectors at once:
 // the row metadata, the V values and the control flow are shared by them.
 // Two LDS tiles alternate, so one barrier per plane suffices.
-template <typename T, int R, int SX, int TYW, int NV>
+template <typename T, int R, int SX, int TYW, int NV, int NB>
 __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
                                                       T* Out, const T* __restrict__ V, T ca, T cb, T cc,
                                                       double* __restrict__ dot_part) {
@@ -88,7 +88,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   constexpr int LW = NXP + 2 * R;   // LDS row stride (R zero columns each side)
   constexpr int LR = TY + 2 * R;    // LDS rows
   constexpr int HT = (2 * R + NW - 1) / NW;  // halo rows per wave
-  __shared__ T tile[2][NV][LR * LW];
+  __shared__ T tile[NB][NV][LR * LW];
   __shared__ double red[NW];
 
   const int bid = blockIdx.x;
@@ -133,7 +133,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
     }
     return;
   }
-  for (int t = threadIdx.x; t < 2 * NV * LR * LW; t += 256) (&tile[0][0][0])[t] = T(0);
+  for (int t = threadIdx.x; t < NB * NV * LR * LW; t += 256) (&tile[0][0][0])[t] = T(0);
   __syncthreads();
 
   auto rowmeta = [&](int j, int z) -> int2 {
@@ -200,7 +200,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   prefetch(zb);
 
   for (int z = zb; z < ze; ++z) {
-    const int b = (z - zb) & 1;
+    const int b = NB == 2 ? ((z - zb) & 1) : 0;
     // commit the prefetched plane
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
@@ -278,6 +278,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
         }
       }
     }
+    if (NB == 1) lds_barrier();   // one tile: the next plane overwrites it
   }
   if (dot_part) {
     const double s = pc_block_sum(dot, red);
@@ -285,15 +286,15 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   }
 }
 
-template <typename T, int R, int SX, int TYW, int NV>
+template <typename T, int R, int SX, int TYW, int NV, int NB = 2>
 constexpr int zmarch_lds_bytes() {
-  return 2 * NV * (4 * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
+  return NB * NV * (4 * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
 }
 
-template <typename T, int R, int SX, int TYW, int NV>
+template <typename T, int R, int SX, int TYW, int NV, int NB = 2>
 int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                double ca, double cb, double cc, double* dot_part, hipStream_t st) {
-  if constexpr (NV > 1 && zmarch_lds_bytes<T, R, SX, TYW, NV>() > 112 * 1024) {
+  if constexpr (NV > 1 && zmarch_lds_bytes<T, R, SX, TYW, NV, NB>() > 112 * 1024) {
     // too much LDS for several vectors per workgroup: one vector per workgroup
     return launch_cfg<T, R, SX, TYW, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
   } else {
@@ -309,7 +310,7 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
   for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
-  hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV>), dim3(8 * a.nb8), dim3(256), 0, st, a, Y, X, Out, V, T(ca),
+  hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB>), dim3(8 * a.nb8), dim3(256), 0, st, a, Y, X, Out, V, T(ca),
                      T(cb), T(cc), dot_part);
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
@@ -317,9 +318,11 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
 }
 
 // One y-row per wavefront (TYW = 1) and as many vectors per workgroup as keep
-// the two LDS tiles within 56 KB (two workgroups per CU): 4 FP32 vectors at
-// radius 4, 2 FP64 vectors, 2 FP32 vectors at radius 6.  Measured on the
-// Si87H76 and 128^3 workloads against taller slabs and 1..8 vectors.
+// the two LDS tiles within 56 KB: 4 FP32 vectors at radius 4, 2 FP64 vectors,
+// 2 FP32 vectors at radius 6.  Measured on the Si87H76 and 128^3 workloads
+// against taller slabs, 1..8 vectors and a single LDS tile (NB = 1, one more
+// barrier per plane): FP32 Si87H76 1.17 ms here, 1.13 ms for 2 vectors with
+// one tile, 1.37-2.3 ms for the others.
 template <typename T, int R, int SX>
 int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                 double ca, double cb, double cc, double* dot_part, hipStream_t st) {

@@ -39,9 +39,15 @@ struct PointMap {
   const int* prow;    // per point: its row j + ny*k
   long long n;
   long long chunk;    // points per workgroup (multiple of kBlk)
+  int nb, nb8;        // chunks (workgroups with work) and ceil(nb / 8)
   int ny, nz;
   double w[8];        // weights of -lap/h^2; w[0] is the 3D centre weight
 };
+
+// Chunk of this workgroup: consecutive chunks go to the same XCD (workgroup
+// ids are dealt round-robin over the 8 XCDs), so the z-neighbour planes a
+// chunk reads are mostly in that XCD's L2.
+__device__ __forceinline__ int chunk_of(int nb8) { return (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3); }
 
 // Sum of part[0..nb) in a fixed order; the same in every workgroup.
 __device__ __forceinline__ double sum_parts(const double* part, int nb, double* red) {
@@ -100,7 +106,9 @@ __global__ __launch_bounds__(kBlk) void cg_init(PointMap g, const double* __rest
                                                 const double* __restrict__ rhs, double* __restrict__ r,
                                                 double* __restrict__ part_rr) {
   __shared__ double red[kBlk / 64];
-  const long long p0 = (long long)blockIdx.x * g.chunk;
+  const int c = chunk_of(g.nb8);
+  if (c >= g.nb) return;
+  const long long p0 = (long long)c * g.chunk;
   const long long p1 = min(p0 + g.chunk, g.n);
   double s = 0.0;
   for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) {
@@ -111,7 +119,7 @@ __global__ __launch_bounds__(kBlk) void cg_init(PointMap g, const double* __rest
     s += rv * rv;
   }
   const double t = pc_block_sum(s, red);
-  if (threadIdx.x == 0) part_rr[blockIdx.x] = t;
+  if (threadIdx.x == 0) part_rr[c] = t;
 }
 
 // Iteration t (t completed so far).  rr_cur: partials of r_t.r_t; sc slot
@@ -126,12 +134,13 @@ __global__ __launch_bounds__(kBlk) void cg_apply(PointMap g, int t, int maxit, d
                                                  double* sc) {
   __shared__ double red[kBlk / 64];
   if (t > 0 && sc[S_DONE] != 0.0) return;
-  const int nb = gridDim.x;
-  const double rr = sum_parts(part_rr, nb, red);
+  const int c = chunk_of(g.nb8);
+  if (c >= g.nb) return;
+  const double rr = sum_parts(part_rr, g.nb, red);
   const double tol1 = t == 0 ? tol * tol * rr : sc[S_TOL1];
   const bool done = t >= maxit || !(rr > tol1);
   const double beta = t > 0 ? rr / sc[S_RO0 + ((t - 1) & 1)] : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (c == 0 && threadIdx.x == 0) {
     sc[S_RO0 + (t & 1)] = rr;
     if (t == 0) sc[S_TOL1] = tol1;
     sc[S_ITS] = (double)t;
@@ -139,7 +148,7 @@ __global__ __launch_bounds__(kBlk) void cg_apply(PointMap g, int t, int maxit, d
     sc[S_DONE] = done ? 1.0 : 0.0;
   }
   if (done) return;
-  const long long p0 = (long long)blockIdx.x * g.chunk;
+  const long long p0 = (long long)c * g.chunk;
   const long long p1 = min(p0 + g.chunk, g.n);
   double s = 0.0;
   for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) {
@@ -150,20 +159,23 @@ __global__ __launch_bounds__(kBlk) void cg_apply(PointMap g, int t, int maxit, d
     s += qc * acc;
   }
   const double tt = pc_block_sum(s, red);
-  if (threadIdx.x == 0) part_pq[blockIdx.x] = tt;
+  if (threadIdx.x == 0) part_pq[c] = tt;
 }
 
 // alpha = r_t.r_t / p.Ap; x += alpha p; r -= alpha Ap; partials of r_{t+1}.r_{t+1}
-__global__ __launch_bounds__(kBlk) void cg_update(long long n, long long chunk, int t, const double* sc,
+__global__ __launch_bounds__(kBlk) void cg_update(long long n, long long chunk, int nb, int nb8, int t,
+                                                  const double* sc,
                                                   double* __restrict__ x, double* __restrict__ r,
                                                   const double* __restrict__ p, const double* __restrict__ ap,
                                                   const double* __restrict__ part_pq,
                                                   double* __restrict__ part_rr_next) {
   __shared__ double red[kBlk / 64];
   if (sc[S_DONE] != 0.0) return;
-  const double pq = sum_parts(part_pq, gridDim.x, red);
+  const int c = chunk_of(nb8);
+  if (c >= nb) return;
+  const double pq = sum_parts(part_pq, nb, red);
   const double alpha = sc[S_RO0 + (t & 1)] / pq;
-  const long long p0 = (long long)blockIdx.x * chunk;
+  const long long p0 = (long long)c * chunk;
   const long long p1 = min(p0 + chunk, n);
   double s = 0.0;
   for (long long i = p0 + threadIdx.x; i < p1; i += kBlk) {
@@ -173,7 +185,7 @@ __global__ __launch_bounds__(kBlk) void cg_update(long long n, long long chunk, 
     s += rv * rv;
   }
   const double tt = pc_block_sum(s, red);
-  if (threadIdx.x == 0) part_rr_next[blockIdx.x] = tt;
+  if (threadIdx.x == 0) part_rr_next[c] = tt;
 }
 
 struct Work {
@@ -272,6 +284,9 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
   const int nb = (int)std::min<long long>(kMaxNB, blocks_of_pts);
   g.chunk = ((blocks_of_pts + nb - 1) / nb) * kBlk;
   const int nb_used = (int)((n + g.chunk - 1) / g.chunk);
+  g.nb = nb_used;
+  g.nb8 = (nb_used + 7) / 8;
+  const int nb_launch = 8 * g.nb8;
 
   Work w;
   w.r = static_cast<double*>(work);
@@ -283,7 +298,7 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
   w.ppq = w.prr[1] + kMaxNB;
   w.sc = w.ppq + kMaxNB;
 
-  if (L.init(nb_used, st, g, x, rhs, w)) {
+  if (L.init(nb_launch, st, g, x, rhs, w)) {
     pc::set_error("pc_pcg_solve: launch failed");
     return pc::PC_ERR_HIP;
   }
@@ -296,9 +311,9 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
   for (int t0 = 0; t0 <= maxit; t0 += check_every) {
     for (int t = t0; t < t0 + check_every && t <= maxit; ++t) {
       const int mode = t == 0 ? M_FIRST : ((t == 1 && alias_first) ? M_ALIAS : M_NORMAL);
-      status = L.apply(mode, nb_used, st, g, t, maxit, tol, w);
+      status = L.apply(mode, nb_launch, st, g, t, maxit, tol, w);
       if (!status && t < maxit) {
-        hipLaunchKernelGGL(cg_update, dim3(nb_used), dim3(kBlk), 0, st, n, g.chunk, t, w.sc, x, w.r,
+        hipLaunchKernelGGL(cg_update, dim3(nb_launch), dim3(kBlk), 0, st, n, g.chunk, g.nb, g.nb8, t, w.sc, x, w.r,
                            w.p[(t + 1) & 1], w.ap, w.ppq, w.prr[(t + 1) & 1]);
         if (hipGetLastError() != hipSuccess) status = pc::PC_ERR_HIP;
       }

@@ -443,7 +443,7 @@ def train_main(argv=None):
     ap.add_argument("--alkanes", type=int, default=400)
     ap.add_argument("--si", type=int, default=100)
     ap.add_argument("--epochs", type=int, default=20)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--gen-budget-s", type=float, default=480.0, help="stop generating data after this long")
     ap.add_argument("--train-budget-s", type=float, default=300.0)

@@ -54,12 +54,13 @@ class SCFConfig:
     pcg_aliasing: bool = True
     pcg_maxit: int = 200
     pcg_tol: float = 1e-5
-    # mixed precision: optionally switch the wave functions to FP64 once |dE|
-    # falls below polish_de Ha/atom (or err below polish_err) and finish in
-    # FP64.  Off by default: with FP64-accumulated Gram matrices the FP32
-    # iteration converges to within 1e-7 Ha/atom of the FP64 fixed point.
-    polish: bool = False
-    polish_de: float = 2e-5
+    # mixed precision: switch the wave functions to FP64 once |dE| falls
+    # below polish_de Ha/atom (or err below polish_err) and finish in FP64.
+    # The FP32 iteration's energy noise is ~1e-7 of the total energy (below
+    # 1e-7 Ha/atom for Si87H76, ~2e-6 Ha/atom for H2O), so the last
+    # iterations run in FP64; the convergence test carries across the switch.
+    polish: bool = True
+    polish_de: float = 5e-6
     polish_err: float | None = None
     verbose: bool = False
 
@@ -228,8 +229,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
             eig64.stats["rr"] += eig.stats["rr"]
             eig = eig64
             hist[-1]["switch_to_fp64"] = True
-            e_prev = None  # the convergence test restarts on FP64 iterates
-            continue
+            continue       # the next (FP64) energy is tested against this one
         if cfg.etol_ha_per_atom is not None and de is not None and de < cfg.etol_ha_per_atom:
             converged = True
             break
