@@ -93,6 +93,8 @@ int pc_dense_destroy(void* ctx);
 int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, double* T, int ldt, double* lam_dev,
                 double* lam_host);
 int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
+/* eigensolver used by pc_dense_rr: 0 = divide and conquer (default), 1 = Jacobi */
+int pc_dense_set_method(void* ctx, int method);
 
 /* ---- point-wise SCF kernels --------------------------------------------------
  * density: rho = sum_v occ2[v] W_v^2           (src/rsdft_solver.py:324)

@@ -23,6 +23,9 @@ struct DenseCtx {
   double* out;      // kmax + 4 : eigenvalues then 3 info codes (as doubles)
   int* info;        // 4
   double* host_out; // pinned, kmax + 4
+  int method;       // 0: divide and conquer (syevd), 1: Jacobi (syevj)
+  double* resid;    // syevj: off-diagonal norm at exit
+  int* sweeps;      // syevj: sweeps used
 };
 
 __global__ void symmetrize(double* G, int k, int ldg) {
@@ -74,7 +77,10 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipMalloc(&c->E, (size_t)kmax * sizeof(double)));
   PC_HIP(hipMalloc(&c->out, (size_t)(kmax + 4) * sizeof(double)));
   PC_HIP(hipMalloc(&c->info, 4 * sizeof(int)));
+  PC_HIP(hipMalloc(&c->resid, sizeof(double)));
+  PC_HIP(hipMalloc(&c->sweeps, sizeof(int)));
   PC_HIP(hipHostMalloc(&c->host_out, (size_t)(kmax + 4) * sizeof(double), hipHostMallocDefault));
+  c->method = 0;
   *out_ctx = c;
   return pc::PC_OK;
 }
@@ -84,6 +90,7 @@ int pc_dense_destroy(void* ctx) {
   if (!c) return pc::PC_OK;
   rocblas_destroy_handle(c->h);
   (void)hipFree(c->tmp); (void)hipFree(c->E); (void)hipFree(c->out); (void)hipFree(c->info);
+  (void)hipFree(c->resid); (void)hipFree(c->sweeps);
   (void)hipHostFree(c->host_out);
   delete c;
   return pc::PC_OK;
@@ -111,8 +118,12 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
                       rocblas_diagonal_non_unit, k, k, &one, S, lds, c->tmp, k, Gy, ldg));
   hipLaunchKernelGGL(symmetrize, dim3((k * k + 255) / 256), dim3(256), 0, c->stream, Gy, k, ldg);
   PC_LAUNCH_CHECK();
-  PC_RB(rocsolver_dsyevd(c->h, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg, c->out, c->E,
-                         c->info + 2));
+  if (c->method == 1)
+    PC_RB(rocsolver_dsyevj(c->h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg,
+                           0.0, c->resid, 100, c->sweeps, c->out, c->info + 2));
+  else
+    PC_RB(rocsolver_dsyevd(c->h, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg, c->out, c->E,
+                           c->info + 2));
   // T = R^-1 Q
   PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
                       rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg, T, ldt));
@@ -132,6 +143,17 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   }
   if (lam_host)
     for (int i = 0; i < k; ++i) lam_host[i] = c->host_out[i];
+  return pc::PC_OK;
+}
+
+// Eigensolver of the projected k x k problem: 0 = divide and conquer
+// (rocSOLVER syevd), 1 = cyclic Jacobi (rocSOLVER syevj, tolerance = machine
+// precision, at most 100 sweeps).
+int pc_dense_set_method(void* ctx, int method) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c, "null context");
+  PC_CHECK_ARG(method == 0 || method == 1, "method must be 0 (syevd) or 1 (syevj)");
+  c->method = method;
   return pc::PC_OK;
 }
 

@@ -489,6 +489,13 @@ int pc_internal_grid_points(void* grid, int* ny, int* nz, int* radius, const int
   for (int d = 0; d < 8; ++d) w8[d] = gd->w[d];
   return 0;
 }
+// z-extent per y (min z, max z + 1 of the non-empty rows) and nx, for z-marching kernels
+int pc_internal_grid_zmap(void* grid, int* nx, const int** zrange) {
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  *nx = gd->nx;
+  *zrange = gd->zrange;
+  return 0;
+}
 long long pc_internal_grid_nrow(void* grid) {
   const GridDev* gd = static_cast<const GridDev*>(grid);
   return (long long)gd->ny * gd->nz;

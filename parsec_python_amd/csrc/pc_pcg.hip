@@ -24,6 +24,7 @@
 
 int pc_internal_grid_points(void* grid, int* ny, int* nz, int* radius, const int2** meta, const int** prow,
                             double* w8);
+int pc_internal_grid_zmap(void* grid, int* nx, const int** zrange);
 long long pc_internal_grid_npts(void* grid);
 
 namespace {
@@ -52,7 +53,7 @@ __device__ __forceinline__ int chunk_of(int nb8) { return (blockIdx.x & 7) * nb8
 // Sum of part[0..nb) in a fixed order; the same in every workgroup.
 __device__ __forceinline__ double sum_parts(const double* part, int nb, double* red) {
   double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += kBlk) s += part[b];
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
   double t = pc_block_sum(s, red);
   __shared__ double bc;
   if (threadIdx.x == 0) bc = t;
@@ -122,25 +123,59 @@ __global__ __launch_bounds__(kBlk) void cg_init(PointMap g, const double* __rest
   if (threadIdx.x == 0) part_rr[c] = t;
 }
 
-// Iteration t (t completed so far).  rr_cur: partials of r_t.r_t; sc slot
-// S_RO0 + ((t-1)&1) holds r_{t-1}.r_{t-1} (t >= 1).  Decides convergence,
-// forms p_new = r + beta p_old, writes p_new, A p_new and partials of
-// p_new . A p_new.
-template <int R, int MODE>
-__global__ __launch_bounds__(kBlk) void cg_apply(PointMap g, int t, int maxit, double tol,
-                                                 const double* __restrict__ r, const double* __restrict__ p_old,
-                                                 double* __restrict__ p_new, double* __restrict__ ap,
-                                                 const double* __restrict__ part_rr, double* __restrict__ part_pq,
-                                                 double* sc) {
-  __shared__ double red[kBlk / 64];
+// ---- iteration step t (t iterations completed so far) ------------------------
+// rr partials of r_t.r_t -> convergence decision (t >= maxit or r.r <= tol^2
+// r0.r0) and beta = r_t.r_t / r_{t-1}.r_{t-1} in every workgroup; workgroup 0
+// records the scalars.  Then p_new = r + beta p_old, A p_new and the partials of
+// p_new . A p_new, as a z-march:
+// a workgroup owns 4 y-rows (one per wavefront) and a chunk of zc z-planes of
+// them, and marches z: each lane keeps its points' 2R+1 z-values of p_new in
+// registers, the current plane of the 4 rows plus R halo rows on each side
+// sits in LDS (two tiles, one barrier per plane) for the x/y differences, and
+// the loads for plane z+1 are issued before plane z is computed.  p_new is
+// formed on the fly from r and p_old.  Partials: one per workgroup.
+constexpr int kZW = 4;
+
+struct ZMap {
+  const int2* meta;
+  const int* zrange;   // [2*ny]
+  int nx, ny, nz;
+  int nslab, nzc, zc;  // 4-row slabs, z-chunks per slab, planes per chunk
+  int nwg, nwg8;       // nslab * nzc workgroups, ceil(nwg / 8)
+  double w[8];
+};
+
+__device__ __forceinline__ int2 zrow(const ZMap& g, int j, int z) {
+  return (j >= 0 && j < g.ny && z >= 0 && z < g.nz) ? g.meta[j + g.ny * z] : make_int2(0, 0);
+}
+
+template <int MODE>
+__device__ __forceinline__ double qload(const double* __restrict__ a, const double* __restrict__ b, double beta,
+                                        int2 m, int i) {
+  const int lo = m.y & 0xffff, hi = m.y >> 16;
+  return (i >= lo && i < hi) ? qv<MODE>(a, b, beta, (long long)m.x + i) : 0.0;
+}
+
+template <int R, int SX, int MODE>
+__global__ __launch_bounds__(256) void cg_apply_z(ZMap g, int t, int maxit, double tol, int nb_rr,
+                                                  const double* __restrict__ r, const double* __restrict__ p_old,
+                                                  double* __restrict__ p_new, double* __restrict__ ap,
+                                                  const double* __restrict__ part_rr, double* __restrict__ part_pq,
+                                                  double* sc) {
+  constexpr int TY = kZW;
+  constexpr int LW = 64 * SX + 2 * R;
+  constexpr int LR = TY + 2 * R;
+  constexpr int HT = (2 * R + kZW - 1) / kZW;
+  __shared__ double tile[2][LR * LW];
+  __shared__ double red[kZW];
   if (t > 0 && sc[S_DONE] != 0.0) return;
-  const int c = chunk_of(g.nb8);
-  if (c >= g.nb) return;
-  const double rr = sum_parts(part_rr, g.nb, red);
+  const int L = (blockIdx.x & 7) * g.nwg8 + (blockIdx.x >> 3);
+  if (L >= g.nwg) return;
+  const double rr = sum_parts(part_rr, nb_rr, red);
   const double tol1 = t == 0 ? tol * tol * rr : sc[S_TOL1];
   const bool done = t >= maxit || !(rr > tol1);
   const double beta = t > 0 ? rr / sc[S_RO0 + ((t - 1) & 1)] : 0.0;
-  if (c == 0 && threadIdx.x == 0) {
+  if (L == 0 && threadIdx.x == 0) {
     sc[S_RO0 + (t & 1)] = rr;
     if (t == 0) sc[S_TOL1] = tol1;
     sc[S_ITS] = (double)t;
@@ -148,22 +183,109 @@ __global__ __launch_bounds__(kBlk) void cg_apply(PointMap g, int t, int maxit, d
     sc[S_DONE] = done ? 1.0 : 0.0;
   }
   if (done) return;
-  const long long p0 = (long long)c * g.chunk;
-  const long long p1 = min(p0 + g.chunk, g.n);
-  double s = 0.0;
-  for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) {
-    double qc;
-    const double acc = apply_point<R, MODE>(g, r, p_old, beta, p, qc);
-    p_new[p] = qc;
-    ap[p] = acc;
-    s += qc * acc;
+  const int slab = L % g.nslab;
+  const int zci = L / g.nslab;
+  const int j0 = slab * TY;
+  int zb = g.nz, ze = 0;
+  for (int q = 0; q < TY; ++q) {
+    const int j = j0 + q;
+    if (j < g.ny) {
+      zb = min(zb, g.zrange[2 * j]);
+      ze = max(ze, g.zrange[2 * j + 1]);
+    }
   }
-  const double tt = pc_block_sum(s, red);
-  if (threadIdx.x == 0) part_pq[c] = tt;
+  const int z0 = zb + zci * g.zc;
+  const int z1 = min(ze, z0 + g.zc);
+  double sacc = 0.0;
+  if (z0 < z1) {
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j = j0 + wv;
+    for (int e = threadIdx.x; e < 2 * LR * LW; e += 256) (&tile[0][0])[e] = 0.0;
+    __syncthreads();
+    double q[SX][2 * R + 1];
+    double nq[SX], hc[HT][SX], hn[HT][SX];
+    int2 mcur, mnext;
+#pragma unroll
+    for (int s = 0; s < SX; ++s) {
+      q[s][0] = 0.0;
+#pragma unroll
+      for (int h = 0; h < HT; ++h) hc[h][s] = hn[h][s] = 0.0;
+    }
+#pragma unroll
+    for (int dz = 1; dz <= 2 * R; ++dz) {
+      const int2 m = zrow(g, j, z0 - R - 1 + dz);
+#pragma unroll
+      for (int s = 0; s < SX; ++s) q[s][dz] = qload<MODE>(r, p_old, beta, m, lane + 64 * s);
+    }
+    auto prefetch = [&](int z) {
+      const int2 mq = zrow(g, j, z + R);
+      mnext = zrow(g, j, z);
+#pragma unroll
+      for (int s = 0; s < SX; ++s) nq[s] = qload<MODE>(r, p_old, beta, mq, lane + 64 * s);
+#pragma unroll
+      for (int h = 0; h < HT; ++h) {
+        const int hr = wv + kZW * h;
+        if (hr < 2 * R) {
+          const int tr = hr < R ? hr : hr + TY;
+          const int2 mh = zrow(g, j0 - R + tr, z);
+#pragma unroll
+          for (int s = 0; s < SX; ++s) hn[h][s] = qload<MODE>(r, p_old, beta, mh, lane + 64 * s);
+        }
+      }
+    };
+    prefetch(z0);
+    for (int z = z0; z < z1; ++z) {
+      double* tb = tile[(z - z0) & 1];
+      mcur = mnext;
+#pragma unroll
+      for (int s = 0; s < SX; ++s) {
+#pragma unroll
+        for (int d = 0; d < 2 * R; ++d) q[s][d] = q[s][d + 1];
+        q[s][2 * R] = nq[s];
+#pragma unroll
+        for (int h = 0; h < HT; ++h) hc[h][s] = hn[h][s];
+      }
+      if (z + 1 < z1) prefetch(z + 1);
+#pragma unroll
+      for (int s = 0; s < SX; ++s) tb[(wv + R) * LW + R + lane + 64 * s] = q[s][R];
+#pragma unroll
+      for (int h = 0; h < HT; ++h) {
+        const int hr = wv + kZW * h;
+        if (hr < 2 * R) {
+          const int tr = hr < R ? hr : hr + TY;
+#pragma unroll
+          for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = hc[h][s];
+        }
+      }
+      __builtin_amdgcn_s_waitcnt(0xc07f);   // LDS only: the prefetch stays in flight
+      __builtin_amdgcn_s_barrier();
+      const int lo = mcur.y & 0xffff, hi = mcur.y >> 16;
+#pragma unroll
+      for (int s = 0; s < SX; ++s) {
+        const int i = lane + 64 * s;
+        const double* trow = tb + (wv + R) * LW + R + i;
+        double acc = g.w[0] * q[s][R];
+#pragma unroll
+        for (int d = 1; d <= R; ++d) {
+          const double nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[s][R - d] + q[s][R + d];
+          acc += g.w[d] * nb;
+        }
+        if (i >= lo && i < hi) {
+          const long long idx = (long long)mcur.x + i;
+          p_new[idx] = q[s][R];
+          ap[idx] = acc;
+          sacc += q[s][R] * acc;
+        }
+      }
+    }
+  }
+  const double bs = pc_block_sum(sacc, red);
+  if (threadIdx.x == 0) part_pq[L] = bs;
 }
 
 // alpha = r_t.r_t / p.Ap; x += alpha p; r -= alpha Ap; partials of r_{t+1}.r_{t+1}
-__global__ __launch_bounds__(kBlk) void cg_update(long long n, long long chunk, int nb, int nb8, int t,
+__global__ __launch_bounds__(kBlk) void cg_update(long long n, long long chunk, int nb, int nb8, int npq, int t,
                                                   const double* sc,
                                                   double* __restrict__ x, double* __restrict__ r,
                                                   const double* __restrict__ p, const double* __restrict__ ap,
@@ -173,7 +295,7 @@ __global__ __launch_bounds__(kBlk) void cg_update(long long n, long long chunk, 
   if (sc[S_DONE] != 0.0) return;
   const int c = chunk_of(nb8);
   if (c >= nb) return;
-  const double pq = sum_parts(part_pq, nb, red);
+  const double pq = sum_parts(part_pq, npq, red);
   const double alpha = sc[S_RO0 + (t & 1)] / pq;
   const long long p0 = (long long)c * chunk;
   const long long p1 = min(p0 + chunk, n);
@@ -207,23 +329,24 @@ int launch_init(int nb, hipStream_t st, const PointMap& g, const double* x, cons
   return hipGetLastError() == hipSuccess ? pc::PC_OK : pc::PC_ERR_HIP;
 }
 
-template <int R>
-int launch_apply(int mode, int nb, hipStream_t st, const PointMap& g, int t, int maxit, double tol, const Work& w) {
+template <int R, int SX>
+int launch_apply_z(int mode, hipStream_t st, const ZMap& z, int t, int maxit, double tol, int nb_rr, const Work& w) {
   const double* p_old = w.p[t & 1];
   double* p_new = w.p[(t + 1) & 1];
   const double* prr = w.prr[t & 1];
+  const dim3 grid(8 * z.nwg8);
   switch (mode) {
     case M_FIRST:
-      hipLaunchKernelGGL((cg_apply<R, M_FIRST>), dim3(nb), dim3(kBlk), 0, st, g, t, maxit, tol, w.r, p_old, p_new,
-                         w.ap, prr, w.ppq, w.sc);
+      hipLaunchKernelGGL((cg_apply_z<R, SX, M_FIRST>), grid, dim3(256), 0, st, z, t, maxit, tol, nb_rr, w.r, p_old,
+                         p_new, w.ap, prr, w.ppq, w.sc);
       break;
     case M_ALIAS:
-      hipLaunchKernelGGL((cg_apply<R, M_ALIAS>), dim3(nb), dim3(kBlk), 0, st, g, t, maxit, tol, w.r, p_old, p_new,
-                         w.ap, prr, w.ppq, w.sc);
+      hipLaunchKernelGGL((cg_apply_z<R, SX, M_ALIAS>), grid, dim3(256), 0, st, z, t, maxit, tol, nb_rr, w.r, p_old,
+                         p_new, w.ap, prr, w.ppq, w.sc);
       break;
     default:
-      hipLaunchKernelGGL((cg_apply<R, M_NORMAL>), dim3(nb), dim3(kBlk), 0, st, g, t, maxit, tol, w.r, p_old, p_new,
-                         w.ap, prr, w.ppq, w.sc);
+      hipLaunchKernelGGL((cg_apply_z<R, SX, M_NORMAL>), grid, dim3(256), 0, st, z, t, maxit, tol, nb_rr, w.r, p_old,
+                         p_new, w.ap, prr, w.ppq, w.sc);
       break;
   }
   return hipGetLastError() == hipSuccess ? pc::PC_OK : pc::PC_ERR_HIP;
@@ -231,12 +354,14 @@ int launch_apply(int mode, int nb, hipStream_t st, const PointMap& g, int t, int
 
 struct Launchers {
   int (*init)(int, hipStream_t, const PointMap&, const double*, const double*, const Work&);
-  int (*apply)(int, int, hipStream_t, const PointMap&, int, int, double, const Work&);
+  int (*apply)(int, hipStream_t, const ZMap&, int, int, double, int, const Work&);
 };
 
 template <int R>
-Launchers launchers() {
-  return {launch_init<R>, launch_apply<R>};
+Launchers launchers(int sx) {
+  if (sx == 1) return {launch_init<R>, launch_apply_z<R, 1>};
+  if (sx == 2) return {launch_init<R>, launch_apply_z<R, 2>};
+  return {launch_init<R>, launch_apply_z<R, 4>};
 }
 
 }  // namespace
@@ -270,13 +395,27 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
   g.n = n;
   for (int d = 0; d < 8; ++d) g.w[d] = w8[d] * inv_h2;
   g.w[0] *= 3.0;
+  ZMap zm;
+  zm.meta = g.meta;
+  zm.ny = g.ny;
+  zm.nz = g.nz;
+  pc_internal_grid_zmap(grid, &zm.nx, &zm.zrange);
+  for (int d = 0; d < 8; ++d) zm.w[d] = g.w[d];
+  zm.nslab = (zm.ny + kZW - 1) / kZW;
+  zm.zc = 4;
+  while ((long long)zm.nslab * ((zm.nz + zm.zc - 1) / zm.zc) > kMaxNB) ++zm.zc;
+  zm.nzc = (zm.nz + zm.zc - 1) / zm.zc;
+  zm.nwg = zm.nslab * zm.nzc;
+  zm.nwg8 = (zm.nwg + 7) / 8;
+  const int sx = zm.nx <= 64 ? 1 : (zm.nx <= 128 ? 2 : 4);
+  PC_CHECK_ARG(zm.nx <= 256, "nx > 256");
   Launchers L;
   switch (radius) {
-    case 1: L = launchers<1>(); break;
-    case 2: L = launchers<2>(); break;
-    case 3: L = launchers<3>(); break;
-    case 4: L = launchers<4>(); break;
-    case 6: L = launchers<6>(); break;
+    case 1: L = launchers<1>(sx); break;
+    case 2: L = launchers<2>(sx); break;
+    case 3: L = launchers<3>(sx); break;
+    case 4: L = launchers<4>(sx); break;
+    case 6: L = launchers<6>(sx); break;
     default: PC_CHECK_ARG(false, "unsupported stencil radius");
   }
   // workgroups own contiguous chunks (multiples of kBlk points)
@@ -311,9 +450,10 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
   for (int t0 = 0; t0 <= maxit; t0 += check_every) {
     for (int t = t0; t < t0 + check_every && t <= maxit; ++t) {
       const int mode = t == 0 ? M_FIRST : ((t == 1 && alias_first) ? M_ALIAS : M_NORMAL);
-      status = L.apply(mode, nb_launch, st, g, t, maxit, tol, w);
+      status = L.apply(mode, st, zm, t, maxit, tol, nb_used, w);
       if (!status && t < maxit) {
-        hipLaunchKernelGGL(cg_update, dim3(nb_launch), dim3(kBlk), 0, st, n, g.chunk, g.nb, g.nb8, t, w.sc, x, w.r,
+        hipLaunchKernelGGL(cg_update, dim3(nb_launch), dim3(kBlk), 0, st, n, g.chunk, g.nb, g.nb8, zm.nwg, t, w.sc, x,
+                           w.r,
                            w.p[(t + 1) & 1], w.ap, w.ppq, w.prr[(t + 1) & 1]);
         if (hipGetLastError() != hipSuccess) status = pc::PC_ERR_HIP;
       }

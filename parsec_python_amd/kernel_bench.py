@@ -107,14 +107,26 @@ def run_synthetic(dtype=F32, reps: int = 1, n: int = 128, radius_pts: float = 60
         ev[2][0].record()
         es.la.gram(Y, HY, nvec, nvec, G, sym=False)
         ev[2][1].record()
+        # the dense stage with the Jacobi eigensolver, on copies (timed separately)
+        from . import _lib
+        S2, G2 = S.clone(), G.clone()
+        _lib.call("pc_dense_set_method", es.la.dense, 1)
+        ej0, ej1 = _events()
+        ej0.record()
+        _, lam_j = es.la.dense_rr(S2, G2, nvec)
+        ej1.record()
+        _lib.call("pc_dense_set_method", es.la.dense, 0)
         ev[3][0].record()
         T, lam = es.la.dense_rr(S, G, nvec)
         ev[3][1].record()
+        torch.cuda.synchronize()
+        times.setdefault("dense_rr_jacobi", []).append(ej0.elapsed_time(ej1))
+        out["jacobi_vs_dc_max_ritz_diff"] = float(np.max(np.abs(np.asarray(lam_j) - np.asarray(lam))))
         ev[4][0].record()
         es.la.rotate(Y, T, nvec, nvec, HY)
         ev[4][1].record()
         torch.cuda.synchronize()
-        for key, (a, b) in zip(times, ev):
+        for key, (a, b) in zip(["filter", "gram_sym", "gram", "dense_rr", "rotate"], ev):
             times[key].append(a.elapsed_time(b))
     ms = {k: float(np.min(v)) for k, v in times.items()}
     fbytes = g.npts * nvec * s * (2 + 3 * (degree - 1))

@@ -310,7 +310,7 @@ class UNet:
         n = y.numel()
         _lib.call("pc_unet_mse", ptr(y), ptr(t.contiguous()), n, ptr(b["dy"]), ptr(b["loss"]), ptr(b["work"]),
                   b["work"].numel(), stream())
-        return b["loss"] / n
+        return b["loss"][0] / n
 
     def adam_step(self, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
         self.step_count += 1
@@ -392,7 +392,7 @@ def sample_geometry(kind: str, rng: np.random.Generator):
     if kind == "alkane":
         return M.alkane(int(rng.integers(1, 9)), rng)
     if kind == "si":
-        return distorted_si_cluster(int(rng.choice([5, 17, 29, 35])), rng)
+        return distorted_si_cluster(int(rng.choice([5, 17, 35])), rng)
     raise ValueError(kind)
 
 

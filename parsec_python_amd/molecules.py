@@ -130,7 +130,7 @@ def silicon_cluster(a=5.43, sih=1.48, n_si=87):
     shells = np.unique(np.round(d[order], 4))
     chosen = None
     for rcut in shells:
-        sel = pts[d <= rcut + 1e-6]
+        sel = pts[d <= rcut + 1e-3]
         if len(sel) >= n_si:
             chosen = sel
             break

@@ -271,6 +271,32 @@ def test_pcg_vs_oracle(gpu, kind, alias):
     dg.close()
 
 
+@pytest.mark.parametrize("nx,ny,nz,kind,radius", [(70, 70, 70, "sphere", 4), (130, 9, 7, "cube", 6),
+                                                  (40, 13, 22, "cube", 2), (66, 66, 66, "sphere", 1)])
+def test_pcg_shapes_vs_oracle(gpu, nx, ny, nz, kind, radius):
+    """Iteration counts and solutions on ragged and wide grids (x widths of 1, 2 and 4 wavefronts, slabs and
+    z-chunks that do not divide the grid)."""
+    import torch
+    from parsec_python_amd.grid import DeviceGrid
+    from parsec_python_amd.hartree import HartreeSolver
+    print(f"pcg shape {nx}x{ny}x{nz} {kind} r={radius}", flush=True)
+    h = 0.35
+    g = _grid(nx, ny, nz, h, kind)
+    n = g.npts
+    A = (1.0 / h ** 2) * ofd.laplacian_masked(g.nx, g.ny, g.nz, radius, g.mask)
+    rng = np.random.default_rng(nx + ny + nz)
+    rhs = rng.standard_normal(n)
+    x0 = np.zeros(n)
+    xr, its_r = opcg(A, rhs, x0, 60, 1e-5, reference_aliasing=True)
+    dg = DeviceGrid(g, radius)
+    hs = HartreeSolver(dg, h, reference_aliasing=True, check_every=16)
+    x = torch.tensor(x0, device="cuda")
+    its = hs.solve(torch.tensor(rhs, device="cuda"), x, 60, 1e-5)
+    assert its == its_r
+    assert np.max(np.abs(x.cpu().numpy() - xr)) <= 1e-9 * np.abs(xr).max()
+    dg.close()
+
+
 @pytest.mark.parametrize("kind", ["cube", "sphere"])
 def test_sad_eval_vs_oracle(gpu, kind):
     from parsec_python_amd import ionic, molecules as M
