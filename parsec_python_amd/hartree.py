@@ -41,4 +41,6 @@ class HartreeSolver:
         self.last_res = res[0]
         # relative residual |r|/|r0| reached (tol^2 r0.r0 was stored)
         self.last_rel = (res[0] * tol * tol / res[1]) ** 0.5 if res[1] > 0 else 0.0
+        # initial residual norm (tol^2 * r0.r0 was stored)
+        self.last_r0 = (res[1] ** 0.5) / tol if tol > 0 else float("nan")
         return its.value

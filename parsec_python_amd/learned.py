@@ -402,7 +402,7 @@ def scf_pair(geo, precision="mixed", etol=1e-6, domain="cube", max_wall=None):
     from .scf import SCFConfig, SCFProblem, run_scf
     atoms, dom, nev = M.prepare(geo, h_ang=TRAIN_H_ANG, radius_ang=TRAIN_RADIUS_ANG)
     cfg = SCFConfig(tol=1e-12, maxits=60, domain=domain, precision=precision, etol_ha_per_atom=etol,
-                    pcg_maxit=1000, seed=1)
+                    pcg_maxit=1000, pcg_aliasing=False, seed=1)
     prob = SCFProblem(atoms, dom, nev, cfg)
     res = run_scf(prob, max_wall=max_wall)
     return prob, res
@@ -423,7 +423,7 @@ def iterations_with(geo, net: UNet | None, precision="mixed", etol=1e-6, domain=
     from .scf import SCFConfig, SCFProblem, run_scf
     atoms, dom, nev = M.prepare(geo, h_ang=TRAIN_H_ANG, radius_ang=TRAIN_RADIUS_ANG)
     cfg = SCFConfig(tol=1e-12, maxits=60, domain=domain, precision=precision, etol_ha_per_atom=etol,
-                    pcg_maxit=1000, seed=1)
+                    pcg_maxit=1000, pcg_aliasing=False, seed=1)
     prob = SCFProblem(atoms, dom, nev, cfg)
     if net is None:
         res = run_scf(prob)

@@ -243,8 +243,10 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         tim["post"] += t3 - t2
         if cfg.verbose:
             b = getattr(eig, "last_bounds", None)
+            bnorm = float(torch.linalg.norm(rhs).item())
             print(f"  SCF {its:3d}  err {err:9.2e}  E {e['total_ry']:.8f} Ry  dE {de if de is not None else 0:.2e}"
-                  f" Ha/atom  CG {cg_its} rel {hart.last_rel:.1e}"
+                  f" Ha/atom  CG {cg_its} rel {hart.last_rel:.1e} r0/|b| {hart.last_r0 / bnorm:.1e}"
+                  f" r/|b| {hart.last_res ** 0.5 / bnorm:.1e}"
                   + (f"  bounds {b[0]:.4f} {b[1]:.4f} {b[2]:.3f}" if b else ""))
         if H.dtype == F32 and cfg.polish and ((de is not None and de < cfg.polish_de) or
                                               (cfg.polish_err is not None and err < cfg.polish_err)):

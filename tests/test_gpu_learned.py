@@ -56,7 +56,7 @@ def test_learned_start_is_closer_to_the_converged_density(gpu):
     for kind in ("alkane", "alkane", "alkane", "si"):
         atoms, dom, nev = M.prepare(sample_geometry(kind, rng), h_ang=TRAIN_H_ANG, radius_ang=TRAIN_RADIUS_ANG)
         cfg = SCFConfig(tol=1e-12, maxits=60, domain="cube", precision="mixed", etol_ha_per_atom=1e-6,
-                        pcg_maxit=1000, seed=1)
+                        pcg_maxit=1000, pcg_aliasing=False, seed=1)
         prob = SCFProblem(atoms, dom, nev, cfg)
         r0 = run_scf(prob)
         rho_c = r0.rho * prob.h ** 3
