@@ -156,15 +156,18 @@ __global__ __launch_bounds__(256) void gram_kernel(const T* __restrict__ X, long
       }
 }
 
+// 16 workgroups per 64x64 tile, one element per thread; chunks summed in
+// order (deterministic)
 __global__ void gram_reduce(const double* __restrict__ part, int ntile, int nchunk, int ti_n, int tj_n, int kx,
                             int ky, double* __restrict__ S, long long lds, int sym) {
-  const int tile = blockIdx.x;
+  const int tile = blockIdx.x >> 4;
   int ti, tj;
   tile_coords(tile, ti_n, tj_n, sym, ti, tj);
-  for (int e = threadIdx.x; e < 4096; e += blockDim.x) {
+  {
+    const int e = (blockIdx.x & 15) * 256 + threadIdx.x;
     const int ii = e & 63, jj = e >> 6;
     const int i = ti * 64 + ii, j = tj * 64 + jj;
-    if (i >= kx || j >= ky) continue;
+    if (i >= kx || j >= ky) return;
     double s = 0.0;
     for (int c = 0; c < nchunk; ++c) s += part[((long long)c * ntile + tile) * 4096 + e];
     S[(long long)j * lds + i] = s;
@@ -282,8 +285,8 @@ int gram_impl(const T* X, long long ldx, const T* Y, long long ldy, long long n,
     hipLaunchKernelGGL((gram_kernel<T, false>), grid, dim3(256), 0, st, X, ldx, Y, ldy, n, kx, ky, p.ti_n, p.tj_n,
                        p.ntile, sym, p.nchunk, p.chunk, part);
   PC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(gram_reduce, dim3(p.ntile), dim3(256), 0, st, part, p.ntile, p.nchunk, p.ti_n, p.tj_n, kx, ky, S,
-                     lds, sym);
+  hipLaunchKernelGGL(gram_reduce, dim3(16 * p.ntile), dim3(256), 0, st, part, p.ntile, p.nchunk, p.ti_n, p.tj_n, kx,
+                     ky, S, lds, sym);
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
 }

@@ -28,49 +28,85 @@ struct ProjDev {
   int* upt_ent;    // [nent] entry ids grouped by point
 };
 
+// C[v][3a + m] = sum_e u_m[e] * Y[v][pt[e]] (FP64 accumulation).  Workgroup =
+// one atom x 16 vectors; each wavefront walks the atom's entries once and
+// applies them to 4 vectors, so the entry tables are read once per 4 vectors.
+constexpr int kGV = 4;   // vectors per wavefront in the gather
 template <typename T>
 __global__ __launch_bounds__(256) void proj_gather(ProjDev P, const T* __restrict__ Y, long long ld, int nvec,
                                                    double* __restrict__ C) {
   const int a = blockIdx.x;
-  const int v = blockIdx.y * 4 + (threadIdx.x >> 6);
+  const int v0 = (blockIdx.y * 4 + (threadIdx.x >> 6)) * kGV;
   const int lane = threadIdx.x & 63;
-  if (v >= nvec) return;
-  const T* yv = Y + (long long)v * ld;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  if (v0 >= nvec) return;
+  double s[kGV][3];
+#pragma unroll
+  for (int u = 0; u < kGV; ++u) s[u][0] = s[u][1] = s[u][2] = 0.0;
   const int e0 = P.atom_ptr[a], e1 = P.atom_ptr[a + 1];
   for (int e = e0 + lane; e < e1; e += 64) {
-    const double y = (double)yv[P.ent_pt[e]];
-    s0 += P.ent_u[3 * e] * y;
-    s1 += P.ent_u[3 * e + 1] * y;
-    s2 += P.ent_u[3 * e + 2] * y;
+    const long long pt = P.ent_pt[e];
+    const double u0 = P.ent_u[3 * e], u1 = P.ent_u[3 * e + 1], u2 = P.ent_u[3 * e + 2];
+#pragma unroll
+    for (int u = 0; u < kGV; ++u) {
+      const int v = min(v0 + u, nvec - 1);
+      const double y = (double)Y[(long long)v * ld + pt];
+      s[u][0] += u0 * y;
+      s[u][1] += u1 * y;
+      s[u][2] += u2 * y;
+    }
   }
-  for (int o = 32; o > 0; o >>= 1) {
-    s0 += __shfl_down(s0, o, 64);
-    s1 += __shfl_down(s1, o, 64);
-    s2 += __shfl_down(s2, o, 64);
-  }
+#pragma unroll
+  for (int u = 0; u < kGV; ++u)
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      for (int o = 32; o > 0; o >>= 1) s[u][m] += __shfl_down(s[u][m], o, 64);
   if (lane == 0) {
-    double* cv = C + (long long)v * 3 * P.natoms + 3 * a;
-    cv[0] = s0; cv[1] = s1; cv[2] = s2;
+#pragma unroll
+    for (int u = 0; u < kGV; ++u) {
+      if (v0 + u >= nvec) break;
+      double* cv = C + (long long)(v0 + u) * 3 * P.natoms + 3 * a;
+      cv[0] = s[u][0]; cv[1] = s[u][1]; cv[2] = s[u][2];
+    }
   }
 }
 
+// Out[v][upt[q]] += alpha * sum over the entries e at point q of
+// scale[atom] * (u[e] . C[v][atom]).  One thread per unique point and 8
+// vectors: the point's entry list is read once for the 8.
+constexpr int kSV = 8;   // vectors per thread in the scatter
+constexpr int kMaxEnt = 8;
 template <typename T>
 __global__ __launch_bounds__(256) void proj_scatter(ProjDev P, const double* __restrict__ C, T* Out, long long ld,
-                                                    double alpha) {
+                                                    int nvec, double alpha) {
   const int q = blockIdx.x * 256 + threadIdx.x;
-  const int v = blockIdx.y;
+  const int v0 = blockIdx.y * kSV;
   if (q >= P.nupts) return;
-  const double* cv = C + (long long)v * 3 * P.natoms;
-  double s = 0.0;
-  for (int t = P.upt_ptr[q]; t < P.upt_ptr[q + 1]; ++t) {
-    const int e = P.upt_ent[t];
-    const int a = P.ent_atom[e];
-    const double* c = cv + 3 * a;
-    s += P.scale[a] * (P.ent_u[3 * e] * c[0] + P.ent_u[3 * e + 1] * c[1] + P.ent_u[3 * e + 2] * c[2]);
+  const int t0 = P.upt_ptr[q], t1 = P.upt_ptr[q + 1];
+  const long long pt = P.upt[q];
+  double acc[kSV];
+#pragma unroll
+  for (int u = 0; u < kSV; ++u) acc[u] = 0.0;
+  for (int tb = t0; tb < t1; tb += kMaxEnt) {
+    const int tn = min(kMaxEnt, t1 - tb);
+    for (int t = 0; t < tn; ++t) {
+      const int e = P.upt_ent[tb + t];
+      const int a = P.ent_atom[e];
+      const double sc = P.scale[a];
+      const double u0 = sc * P.ent_u[3 * e], u1 = sc * P.ent_u[3 * e + 1], u2 = sc * P.ent_u[3 * e + 2];
+#pragma unroll
+      for (int u = 0; u < kSV; ++u) {
+        const int v = min(v0 + u, nvec - 1);
+        const double* c = C + (long long)v * 3 * P.natoms + 3 * a;
+        acc[u] += u0 * c[0] + u1 * c[1] + u2 * c[2];
+      }
+    }
   }
-  T* o = Out + (long long)v * ld + P.upt[q];
-  *o = (T)((double)(*o) + alpha * s);
+#pragma unroll
+  for (int u = 0; u < kSV; ++u) {
+    if (v0 + u >= nvec) break;
+    T* o = Out + (long long)(v0 + u) * ld + pt;
+    *o = (T)((double)(*o) + alpha * acc[u]);
+  }
 }
 
 template <typename T>
@@ -78,7 +114,7 @@ int gather_impl(void* proj, const T* Y, long long ld, int nvec, double* C, void*
   const ProjDev* P = static_cast<const ProjDev*>(proj);
   PC_CHECK_ARG(P && Y && C, "null pointer");
   if (P->natoms == 0 || nvec == 0) return pc::PC_OK;
-  dim3 grid(P->natoms, (nvec + 3) / 4);
+  dim3 grid(P->natoms, (nvec + 4 * kGV - 1) / (4 * kGV));
   hipLaunchKernelGGL(proj_gather<T>, grid, dim3(256), 0, static_cast<hipStream_t>(stream), *P, Y, ld, nvec, C);
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
@@ -89,9 +125,9 @@ int scatter_impl(void* proj, const double* C, T* Out, long long ld, int nvec, do
   const ProjDev* P = static_cast<const ProjDev*>(proj);
   PC_CHECK_ARG(P && C && Out, "null pointer");
   if (P->nupts == 0 || nvec == 0) return pc::PC_OK;
-  PC_CHECK_ARG(nvec <= 65535, "nvec > 65535");
-  dim3 grid((P->nupts + 255) / 256, nvec);
-  hipLaunchKernelGGL(proj_scatter<T>, grid, dim3(256), 0, static_cast<hipStream_t>(stream), *P, C, Out, ld, alpha);
+  dim3 grid((P->nupts + 255) / 256, (nvec + kSV - 1) / kSV);
+  hipLaunchKernelGGL(proj_scatter<T>, grid, dim3(256), 0, static_cast<hipStream_t>(stream), *P, C, Out, ld, nvec,
+                     alpha);
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
 }
