@@ -78,7 +78,7 @@ __device__ __forceinline__ T meta_val(const T* __restrict__ y, int2 m, int i) {
 // V and x) are already in flight.  A workgroup handles NV vectors at once:
 // the row metadata, the V values and the control flow are shared by them.
 // Two LDS tiles alternate, so one barrier per plane suffices.
-template <typename T, int R, int SX, int TYW, int NV, int NB>
+template <typename T, int R, int SX, int TYW, int NV, int NB, bool HASX, bool DOT>
 __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
                                                       T* Out, const T* __restrict__ V, T ca, T cb, T cc,
                                                       double* __restrict__ dot_part) {
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
     xv[u] = X ? X + base : nullptr;
     ov[u] = Out + base;
   }
-  const bool hasx = X != nullptr;
+  constexpr bool hasx = HASX;
 
   int zb = nz, ze = 0;
   for (int t = 0; t < TY; ++t) {
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   }
   double dot = 0.0;
   if (zb >= ze) {
-    if (dot_part) {
+    if constexpr (DOT) {
       const double s = pc_block_sum(0.0, red);
       if (threadIdx.x == 0) dot_part[L] = s;
     }
@@ -255,32 +255,32 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
 #pragma unroll
       for (int s = 0; s < SX; ++s) {
         const int i = lane + 64 * s;
-        if (i >= lo && i < hi) {
-          const T va = ca * vq[r][s] + cb;
+        // computed on every lane (values outside the domain are zeros), stored inside it
+        const bool in = i >= lo && i < hi;
+        const T va = ca * vq[r][s] + cb;
 #pragma unroll
-          for (int u = 0; u < NV; ++u) {
-            const T yc = q[u][r][s][R];
-            T acc = T(g.w[0]) * yc;
-            const T* trow = tile[b][u] + tr * LW + R + i;
+        for (int u = 0; u < NV; ++u) {
+          const T yc = q[u][r][s][R];
+          T acc = T(g.w[0]) * yc;
+          const T* trow = tile[b][u] + tr * LW + R + i;
 #pragma unroll
-            for (int d = 1; d <= R; ++d) {
-              const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[u][r][s][R - d] +
-                           q[u][r][s][R + d];
-              acc += T(g.w[d]) * nb;
-            }
-            T o = acc + va * yc;
-            if (hasx) o += cc * xq[u][r][s];
-            if (act[u]) {
-              ov[u][mo.x + i] = o;
-              if (dot_part) dot += (double)o * (double)yc;
-            }
+          for (int d = 1; d <= R; ++d) {
+            const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[u][r][s][R - d] +
+                         q[u][r][s][R + d];
+            acc += T(g.w[d]) * nb;
+          }
+          T o = acc + va * yc;
+          if constexpr (HASX) o += cc * xq[u][r][s];
+          if (in && act[u]) {
+            ov[u][mo.x + i] = o;
+            if constexpr (DOT) dot += (double)o * (double)yc;
           }
         }
       }
     }
     if (NB == 1) lds_barrier();   // one tile: the next plane overwrites it
   }
-  if (dot_part) {
+  if constexpr (DOT) {
     const double s = pc_block_sum(dot, red);
     if (threadIdx.x == 0) dot_part[L] = s;
   }
@@ -310,8 +310,26 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
   for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
-  hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB>), dim3(8 * a.nb8), dim3(256), 0, st, a, Y, X, Out, V, T(ca),
-                     T(cb), T(cc), dot_part);
+  const dim3 grid(8 * a.nb8);
+  if (dot_part) {
+    if constexpr (NV == 1) {
+      if (X)
+        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, true>), grid, dim3(256), 0, st, a, Y, X, Out,
+                           V, T(ca), T(cb), T(cc), dot_part);
+      else
+        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, true>), grid, dim3(256), 0, st, a, Y, X,
+                           Out, V, T(ca), T(cb), T(cc), dot_part);
+    } else {
+      pc::set_error("stencil: partial sums need one vector per workgroup");
+      return pc::PC_ERR_ARG;
+    }
+  } else if (X) {
+    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, false>), grid, dim3(256), 0, st, a, Y, X, Out, V,
+                       T(ca), T(cb), T(cc), dot_part);
+  } else {
+    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, false>), grid, dim3(256), 0, st, a, Y, X, Out,
+                       V, T(ca), T(cb), T(cc), dot_part);
+  }
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
   }
