@@ -186,7 +186,9 @@ class Eigensolver:
         S = self.S.view(-1)[:k * k]
         G = self.G.view(-1)[:k * k]
         self.la.gram(Y, Y, k, k, S, sym=True)
-        self.la.gram(Y, HY, k, k, G, sym=False)
+        # G = Y^T (H Y) is symmetric: only the upper tiles are computed and mirrored
+        # (the dense stage symmetrises G again after the Cholesky transform)
+        self.la.gram(Y, HY, k, k, G, sym=True)
         T, lam = self.la.dense_rr(S, G, k)
         W = HY  # HY is no longer needed
         self.la.rotate(Y, T, k, k, W)
