@@ -141,6 +141,17 @@ int pc_pcg_workspace(void* grid, size_t* out_bytes);
 int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
                  int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream);
 
+/* Multigrid-preconditioned CG on the same system and stop rule (replaces the
+ * reference's optional preconditioned CG, cg_prec in src/Eigensolvers/pcg.py:7-85,
+ * whose ILU preconditioner has no GPU form): one symmetric cell-centred V-cycle
+ * per iteration (damped Jacobi on the domain's stencil and on 7-point cube
+ * levels, Chebyshev on the coarsest cube).  Textbook PCG, no aliasing. */
+int pc_mg_create(void* grid, double h, void** out_mg);
+int pc_mg_destroy(void* mg);
+int pc_mg_workspace(void* mg, size_t* out_bytes);
+int pc_pcg_solve_mg(void* mg, const double* rhs, double* x, int maxit, double tol, int check_every, void* work,
+                    size_t work_bytes, int* its_out, double* res_out, void* stream);
+
 /* ---- ionic setup ------------------------------------------------------------------
  * replaces: the per-point spline loops of src/V_ion/pseudoDiag.py:199-278
  * (fsplevalIO, src/Splines/fsplevalIO.py:2-65). */

@@ -73,6 +73,10 @@ PROTOTYPES = {
     "pc_rand_f32": [_ll, _i, _ll, _u64, _i, _vp, _vp],
     "pc_pcg_workspace": [_vp, _psz],
     "pc_pcg_solve": [_vp, _d, _vp, _vp, _i, _d, _i, _i, _vp, _sz, _pi, _vp, _vp],
+    "pc_mg_create": [_vp, _d, C.POINTER(_vp)],
+    "pc_mg_destroy": [_vp],
+    "pc_mg_workspace": [_vp, _psz],
+    "pc_pcg_solve_mg": [_vp, _vp, _vp, _i, _d, _i, _vp, _sz, _pi, _vp, _vp],
     "pc_sad_eval": [_vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pc_spline_eval": [_vp, _vp, _vp, _vp, _i, _vp, _ll, _vp, _vp],
     # learned initialiser (3D U-Net) kernels

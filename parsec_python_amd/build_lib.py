@@ -14,7 +14,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libparsec_hip.so")
 SOURCES = ["pc_common.hip", "pc_grid.hip", "pc_proj.hip", "pc_gemm.hip", "pc_dense.hip", "pc_vec.hip",
-           "pc_pcg.hip", "pc_setup.hip", "pc_unet.hip"]
+           "pc_pcg.hip", "pc_mg.hip", "pc_setup.hip", "pc_unet.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("PARSEC_OFFLOAD_ARCH", "gfx950")
 CFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",

@@ -18,13 +18,27 @@ from .grid import DeviceGrid
 
 
 class HartreeSolver:
-    def __init__(self, dgrid: DeviceGrid, h: float, reference_aliasing: bool = True, check_every: int = 16):
+    """``prec="mg"`` selects the multigrid-preconditioned CG (pc_pcg_solve_mg):
+    same system and stop rule, fewer iterations; the reference's optional
+    preconditioned CG (cg_prec) is the counterpart."""
+
+    def __init__(self, dgrid: DeviceGrid, h: float, reference_aliasing: bool = True, check_every: int = 16,
+                 prec: str | None = None):
         self.dgrid = dgrid
         self.inv_h2 = 1.0 / (h * h)
         self.alias = int(bool(reference_aliasing))
         self.check_every = check_every
+        self.mg = None
         nb = C.c_size_t()
-        _lib.call("pc_pcg_workspace", dgrid.handle, C.byref(nb))
+        if prec == "mg":
+            handle = C.c_void_p()
+            _lib.call("pc_mg_create", dgrid.handle, h, C.byref(handle))
+            self.mg = handle
+            _lib.call("pc_mg_workspace", self.mg, C.byref(nb))
+        elif prec in (None, "none"):
+            _lib.call("pc_pcg_workspace", dgrid.handle, C.byref(nb))
+        else:
+            raise ValueError("prec must be None, 'none' or 'mg'")
         self.work_bytes = nb.value
         self.scratch = Scratch()
         self.last_its = 0
@@ -35,8 +49,12 @@ class HartreeSolver:
         work = self.scratch.bytes("pcg", self.work_bytes)
         its = C.c_int()
         res = (C.c_double * 2)()
-        _lib.call("pc_pcg_solve", self.dgrid.handle, self.inv_h2, ptr(rhs), ptr(x), maxit, tol, self.alias,
-                  self.check_every, ptr(work), self.work_bytes, C.byref(its), res, stream())
+        if self.mg is not None:
+            _lib.call("pc_pcg_solve_mg", self.mg, ptr(rhs), ptr(x), maxit, tol, self.check_every, ptr(work),
+                      self.work_bytes, C.byref(its), res, stream())
+        else:
+            _lib.call("pc_pcg_solve", self.dgrid.handle, self.inv_h2, ptr(rhs), ptr(x), maxit, tol, self.alias,
+                      self.check_every, ptr(work), self.work_bytes, C.byref(its), res, stream())
         self.last_its = its.value
         self.last_res = res[0]
         # relative residual |r|/|r0| reached (tol^2 r0.r0 was stored)
@@ -44,3 +62,14 @@ class HartreeSolver:
         # initial residual norm (tol^2 * r0.r0 was stored)
         self.last_r0 = (res[1] ** 0.5) / tol if tol > 0 else float("nan")
         return its.value
+
+    def close(self):
+        if self.mg is not None:
+            _lib.call("pc_mg_destroy", self.mg)
+            self.mg = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -169,7 +169,7 @@ def run_stencil(name: str, dtype=F32, reps: int = 10) -> dict:
             "bytes": nbytes, "GBs": nbytes / (ms * 1e-3) / 1e9}
 
 
-def run_cg(name: str, iters: int = 300, reps: int = 3) -> dict:
+def run_cg(name: str, iters: int = 300, reps: int = 3, prec: str | None = None, tol: float = 0.0) -> dict:
     """Hartree CG on the workload's domain: rhs ~ N(0,1), zero start, fixed
     iteration count (tolerance 0), so the time per iteration is measured."""
     from .grid import DeviceGrid
@@ -177,7 +177,7 @@ def run_cg(name: str, iters: int = 300, reps: int = 3) -> dict:
     require_gpu()
     g, _ = workload_grid(name)
     dg = DeviceGrid(g, 4)
-    hs = HartreeSolver(dg, g.h, reference_aliasing=True, check_every=16)
+    hs = HartreeSolver(dg, g.h, reference_aliasing=prec is None, check_every=16, prec=prec)
     gen = torch.Generator(device="cuda")
     gen.manual_seed(SYNTH_SEED)
     rhs = torch.randn(g.npts, dtype=F64, device="cuda", generator=gen)
@@ -188,12 +188,14 @@ def run_cg(name: str, iters: int = 300, reps: int = 3) -> dict:
         torch.cuda.synchronize()
         e0, e1 = _events()
         e0.record()
-        its = hs.solve(rhs, x, iters, 0.0)
+        its = hs.solve(rhs, x, iters, tol)
         e1.record()
         torch.cuda.synchronize()
         best = min(best, e0.elapsed_time(e1))
+    hs.close()
     dg.close()
-    return {"workload": name, "points": g.npts, "iterations": its, "ms": best, "us_per_iteration": 1e3 * best / its}
+    return {"workload": name, "points": g.npts, "prec": prec or "none", "tol": tol, "iterations": its, "ms": best,
+            "us_per_iteration": 1e3 * best / max(its, 1), "final_rel_residual": hs.last_rel}
 
 
 def run_copy(mb: float = 666.0, reps: int = 3) -> dict:
@@ -219,6 +221,9 @@ def main(argv=None):
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--mb", type=float, default=666.0)
+    ap.add_argument("--prec", default="none", choices=["none", "mg"], help="cg: preconditioner")
+    ap.add_argument("--tol", type=float, default=0.0, help="cg: relative tolerance (0: run --iters iterations)")
+    ap.add_argument("--iters", type=int, default=300, help="cg: iteration cap")
     a = ap.parse_args(argv)
     dt = F32 if a.dtype == "f32" else F64
     if a.what == "synthetic":
@@ -226,7 +231,7 @@ def main(argv=None):
     elif a.what == "stencil":
         res = run_stencil(a.workload, dt, a.reps)
     elif a.what == "cg":
-        res = run_cg(a.workload, 300, a.reps)
+        res = run_cg(a.workload, a.iters, a.reps, None if a.prec == "none" else a.prec, a.tol)
     else:
         res = run_copy(a.mb, a.reps)
     print(json.dumps(res), flush=True)

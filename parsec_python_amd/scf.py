@@ -54,6 +54,9 @@ class SCFConfig:
     pcg_aliasing: bool = True
     pcg_maxit: int = 200
     pcg_tol: float = 1e-5
+    # Hartree CG preconditioner: "none" (the reference default, cg_prec = 0) or
+    # "mg" (multigrid V-cycle; same stop rule, fewer iterations)
+    pcg_prec: str = "none"
     # mixed precision: switch the wave functions to FP64 once |dE| falls
     # below polish_de Ha/atom (or err below polish_err) and finish in FP64.
     # The FP32 iteration's energy noise is ~1e-7 of the total energy (below
@@ -175,7 +178,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     eig = Eigensolver(H, k, seed=cfg.seed)
     dense_method = {"syevd": 0, "syevj": 1}[cfg.dense_method]
     _lib.call("pc_dense_set_method", eig.la.dense, dense_method)
-    hart = HartreeSolver(H.dgrid, h, reference_aliasing=cfg.pcg_aliasing)
+    hart = HartreeSolver(H.dgrid, h, reference_aliasing=cfg.pcg_aliasing, prec=cfg.pcg_prec)
     mixer = DeviceMsecant1(n, cfg.maxits + 2)
 
     err = 10.0 + cfg.tol
