@@ -15,27 +15,41 @@
 // (R = P^T / 8), one damped-Jacobi step before and after the coarse
 // correction on every level, and a fixed-degree Chebyshev iteration on the
 // coarsest cube (a polynomial in A, so the cycle is a fixed symmetric linear
-// operator and CG theory applies).  Each level costs two launches: the
-// pre-smooth, residual and restriction are one kernel (from a zero start
-// the smoothed value is omega b / D, formed on the fly), and the
-// prolongation with the post-smooth is another.
+// operator and CG theory applies).  From a zero start the pre-smoothed value
+// is omega b / D, so the pre-smooth, residual and restriction of a cube level
+// are one kernel, and the prolongation with the post-smooth is another.
+//
+// Launches per iteration:
+//   fine level   mg_res0 (z-march: t0 = r - c0 A r), mg_restrict0,
+//                mg_up0 (z-march: z = post-smooth(c0 r + P e_1), partials of r.z),
+//                mg_apply (z-march: p = z + beta p, A p, partials of p.Ap), mg_update;
+//   cube levels  mg_down / mg_up per level above the tail;
+//   tail         mg_tail: the smallest levels and the coarsest Chebyshev in
+//                one workgroup with every level in LDS.
+// Arrays of a cube level hold zero on inactive cells, so a neighbour read
+// needs no mask.  Every kernel returns at once when the iteration has
+// converged, so the host reads the state back only every check_every
+// iterations.
 #include "pc_common.h"
+#include "pc_zmarch.h"
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 long long pc_internal_grid_npts(void* grid);
-int pc_internal_grid_points(void* grid, int* ny, int* nz, int* radius, const int2** meta, const int** prow,
-                            double* w8);
-int pc_internal_grid_zmap(void* grid, int* nx, const int** zrange);
 int pc_internal_grid_view(void* grid, int* nx, int* ny, int* nz, const int** lo, const int** hi, const int** off);
 
 namespace {
 
+using pcz::ZMap;
+
 constexpr int kB = 256;
-constexpr int kNB = 512;          // workgroups of the point-wise launches (= partials)
+constexpr int kNB = 512;          // workgroups of the point-wise launches (= r.r partials)
 constexpr int kMaxLev = 8;
-constexpr int kCheb = 16;         // Chebyshev steps on the coarsest cube
+constexpr int kCheb = 8;          // Chebyshev steps on the coarsest cube
+constexpr int kCoarsest = 512;    // cells of the coarsest cube, at most
+constexpr int kTailCells = 512;   // cells of the levels held in LDS by mg_tail, at most (measured: one level beats more)
+constexpr int kTailThreads = 1024;
 constexpr double kOmega0 = 0.6;   // damped Jacobi on the domain's stencil (lambda_max(D^-1 A) < 2.3 for fd8)
 constexpr double kOmegaC = 0.8;   // damped Jacobi on the 7-point cubes (lambda_max(D^-1 A) < 2)
 enum { S_RR0 = 0, S_RZ0, S_RZ1, S_TOL1, S_DONE, S_ITS, S_RES, S_COUNT = 8 };
@@ -48,7 +62,7 @@ struct Fine {
   double w[8];        // weights of A (w[0] = centre)
   double c0;          // omega0 / w[0]
   long long chunk;
-  int nb;
+  int nb, nb8;
 };
 
 struct Cube {
@@ -60,13 +74,22 @@ struct Cube {
   double* e;
 };
 
-__device__ __forceinline__ void row_of(const Fine& g, long long p, int& row, int2& m, int& i, int& j, int& k) {
-  row = g.prow[p];
-  m = g.meta[row];
-  i = (int)(p - m.x);
-  k = row / g.ny;
-  j = row - k * g.ny;
+__device__ __forceinline__ double block_sum_bcast(double v, double* red) {
+  const double t = pc_block_sum(v, red);
+  __shared__ double bc;
+  if (threadIdx.x == 0) bc = t;
+  __syncthreads();
+  return bc;
 }
+
+// Sum of part[0..nb) in a fixed order; the same in every workgroup.
+__device__ __forceinline__ double sum_parts(const double* part, int nb, double* red) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
+  return block_sum_bcast(s, red);
+}
+
+__device__ __forceinline__ int chunk_of(int nb8) { return (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3); }
 
 // value of a compact-domain array at cube coordinates (0 outside the domain)
 __device__ __forceinline__ double dom_at(const Fine& g, const double* __restrict__ a, int i, int j, int k) {
@@ -76,222 +99,104 @@ __device__ __forceinline__ double dom_at(const Fine& g, const double* __restrict
   return (i >= lo && i < hi) ? a[(long long)m.x + i] : 0.0;
 }
 
-__device__ __forceinline__ double block_sum_bcast(double v, double* red) {
-  const double t = pc_block_sum(v, red);
-  __shared__ double bc;
-  if (threadIdx.x == 0) bc = t;
-  __syncthreads();
-  return bc;
-}
-
-__device__ __forceinline__ double sum_parts(const double* part, int nb, double* red) {
-  double s = 0.0;
-  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
-  return block_sum_bcast(s, red);
-}
-
-// A applied at point p to a field given by f(i, j, k) (FP64), radius R
-template <int R, typename F>
-__device__ __forceinline__ double apply_A(const Fine& g, int i, int j, int k, double fc, F f) {
-  double acc = g.w[0] * fc;
-#pragma unroll
-  for (int d = 1; d <= R; ++d)
-    acc += g.w[d] * (f(i - d, j, k) + f(i + d, j, k) + f(i, j - d, k) + f(i, j + d, k) + f(i, j, k - d) +
-                     f(i, j, k + d));
-  return acc;
-}
-
 // ---------------------------------------------------------------- fine level
-// r = rhs - A x; partials of r.r
+// r = rhs - A x; partials of r.r (once per solve)
 template <int R>
 __global__ __launch_bounds__(kB) void mg_init(Fine g, const double* __restrict__ x, const double* __restrict__ rhs,
                                               double* __restrict__ r, double* __restrict__ prr) {
   __shared__ double red[kB / 64];
-  const long long p0 = (long long)blockIdx.x * g.chunk, p1 = min(p0 + g.chunk, g.n);
+  const int c = chunk_of(g.nb8);
+  if (c >= g.nb) return;
+  const long long p0 = (long long)c * g.chunk, p1 = min(p0 + g.chunk, g.n);
   double s = 0.0;
   for (long long p = p0 + threadIdx.x; p < p1; p += kB) {
-    int row, i, j, k;
-    int2 m;
-    row_of(g, p, row, m, i, j, k);
-    auto f = [&](int a, int b, int c) { return dom_at(g, x, a, b, c); };
-    const double rv = rhs[p] - apply_A<R>(g, i, j, k, x[p], f);
+    const int row = g.prow[p];
+    const int2 m = g.meta[row];
+    const int i = (int)(p - m.x), k = row / g.ny, j = row - k * g.ny;
+    double acc = g.w[0] * x[p];
+#pragma unroll
+    for (int d = 1; d <= R; ++d)
+      acc += g.w[d] * (dom_at(g, x, i - d, j, k) + dom_at(g, x, i + d, j, k) + dom_at(g, x, i, j - d, k) +
+                       dom_at(g, x, i, j + d, k) + dom_at(g, x, i, j, k - d) + dom_at(g, x, i, j, k + d));
+    const double rv = rhs[p] - acc;
     r[p] = rv;
     s += rv * rv;
   }
   const double t = pc_block_sum(s, red);
-  if (threadIdx.x == 0) prr[blockIdx.x] = t;
+  if (threadIdx.x == 0) prr[c] = t;
 }
 
-// t0 = r - A (c0 r): the residual after one damped-Jacobi step from zero
-template <int R>
-__global__ __launch_bounds__(kB) void mg_res0(Fine g, const double* __restrict__ r, double* __restrict__ t0,
-                                              const double* sc) {
+// t0 = r - c0 A r: the residual after one damped-Jacobi step from zero
+template <int R, int SX>
+__global__ __launch_bounds__(256) void mg_res0(ZMap zm, double c0, const double* __restrict__ r,
+                                               double* __restrict__ t0, const double* sc) {
+  __shared__ double tile[pcz::Tile<R, SX>::SIZE];
   if (sc[S_DONE] != 0.0) return;
-  const long long p = (long long)blockIdx.x * kB + threadIdx.x;
-  if (p >= g.n) return;
-  int row, i, j, k;
-  int2 m;
-  row_of(g, p, row, m, i, j, k);
-  auto f = [&](int a, int b, int c) { return dom_at(g, r, a, b, c); };
-  t0[p] = r[p] - g.c0 * apply_A<R>(g, i, j, k, r[p], f);
+  const int L = pcz::block_of(zm);
+  if (L >= zm.nwg) return;
+  pcz::zmarch<R, SX>(
+      zm, L, tile, [&](long long idx, int, int, int) { return r[idx]; },
+      [&](long long idx, int, int, int, double qc, double acc) { t0[idx] = qc - c0 * acc; });
 }
 
 // level-1 right-hand side: average of the domain children of each cell
-__global__ void mg_restrict0(Fine g, const double* __restrict__ t0, Cube c1, const double* sc) {
+__global__ __launch_bounds__(kB) void mg_restrict0(Fine g, const double* __restrict__ t0, Cube c1, const double* sc) {
   if (sc[S_DONE] != 0.0) return;
   const long long q = (long long)blockIdx.x * kB + threadIdx.x;
   if (q >= c1.cells) return;
   const int I = (int)(q % c1.nx), J = (int)((q / c1.nx) % c1.ny), K = (int)(q / ((long long)c1.nx * c1.ny));
   double s = 0.0;
-  if (c1.mask[q]) {
 #pragma unroll
-    for (int d = 0; d < 8; ++d) s += dom_at(g, t0, 2 * I + (d & 1), 2 * J + ((d >> 1) & 1), 2 * K + (d >> 2));
-  }
-  c1.b[q] = 0.125 * s;
-}
-
-// 7-point value helpers on a cube level
-__device__ __forceinline__ double cube_at(const Cube& c, const double* __restrict__ a, int i, int j, int k) {
-  if (i < 0 || j < 0 || k < 0 || i >= c.nx || j >= c.ny || k >= c.nz) return 0.0;
-  const long long q = ((long long)k * c.ny + j) * c.nx + i;
-  return c.mask[q] ? a[q] : 0.0;
-}
-
-// b_{l+1} = R (b_l - A_l (omega b_l / D_l)) on the active cells
-__global__ void mg_down(Cube f, Cube c, const double* sc) {
-  if (sc[S_DONE] != 0.0) return;
-  const long long q = (long long)blockIdx.x * kB + threadIdx.x;
-  if (q >= c.cells) return;
-  const int I = (int)(q % c.nx), J = (int)((q / c.nx) % c.ny), K = (int)(q / ((long long)c.nx * c.ny));
-  double s = 0.0;
-  if (c.mask[q]) {
-#pragma unroll
-    for (int d = 0; d < 8; ++d) {
-      const int i = 2 * I + (d & 1), j = 2 * J + ((d >> 1) & 1), k = 2 * K + (d >> 2);
-      if (i >= f.nx || j >= f.ny || k >= f.nz) continue;
-      const long long fq = ((long long)k * f.ny + j) * f.nx + i;
-      if (!f.mask[fq]) continue;
-      const double bc = f.b[fq];
-      const double nb = cube_at(f, f.b, i - 1, j, k) + cube_at(f, f.b, i + 1, j, k) + cube_at(f, f.b, i, j - 1, k) +
-                        cube_at(f, f.b, i, j + 1, k) + cube_at(f, f.b, i, j, k - 1) + cube_at(f, f.b, i, j, k + 1);
-      // t = b - (omega/6) (6 b - sum nb)
-      s += (1.0 - kOmegaC) * bc + (kOmegaC / 6.0) * nb;
-    }
-  }
-  c.b[q] = 0.125 * s;
-}
-
-// coarsest cube: e = p(A) b, Chebyshev iteration on [lmin, lmax] from zero,
-// one workgroup, the level in LDS
-__global__ __launch_bounds__(512) void mg_coarsest(Cube c, double lmin, double lmax, const double* sc) {
-  __shared__ double xs[512], rs[512], ds[512];
-  if (sc[S_DONE] != 0.0) return;
-  const int q = threadIdx.x;
-  const bool act = q < c.cells && c.mask[q];
-  const int I = q % c.nx, J = (q / c.nx) % c.ny, K = q / (c.nx * c.ny);
-  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin);
-  const double sigma = theta / delta;
-  double rho = 1.0 / sigma;
-  const double b = act ? c.b[q] : 0.0;
-  xs[q] = 0.0;
-  rs[q] = b;
-  ds[q] = b / theta;
-  __syncthreads();
-  auto at = [&](const double* a, int i, int j, int k) -> double {
-    if (i < 0 || j < 0 || k < 0 || i >= c.nx || j >= c.ny || k >= c.nz) return 0.0;
-    const int qq = (k * c.ny + j) * c.nx + i;
-    return (qq < c.cells && c.mask[qq]) ? a[qq] : 0.0;
-  };
-  for (int it = 0; it < kCheb; ++it) {
-    double x = xs[q] + ds[q];
-    __syncthreads();
-    xs[q] = act ? x : 0.0;
-    __syncthreads();
-    // r = b - A x
-    double r = 0.0;
-    if (act) {
-      const double ax = c.inv_h2 * (6.0 * xs[q] - at(xs, I - 1, J, K) - at(xs, I + 1, J, K) - at(xs, I, J - 1, K) -
-                                    at(xs, I, J + 1, K) - at(xs, I, J, K - 1) - at(xs, I, J, K + 1));
-      r = b - ax;
-    }
-    const double rho1 = 1.0 / (2.0 * sigma - rho);
-    const double d = rho1 * rho * ds[q] + 2.0 * rho1 / delta * r;
-    rho = rho1;
-    __syncthreads();
-    ds[q] = act ? d : 0.0;
-    __syncthreads();
-  }
-  if (q < c.cells) c.e[q] = act ? xs[q] + ds[q] : 0.0;
-}
-
-// e_l = post-smooth(omega b_l / D_l + P e_{l+1}) on the active cells
-__global__ void mg_up(Cube f, Cube c, const double* sc) {
-  if (sc[S_DONE] != 0.0) return;
-  const long long q = (long long)blockIdx.x * kB + threadIdx.x;
-  if (q >= f.cells) return;
-  if (!f.mask[q]) {
-    f.e[q] = 0.0;
-    return;
-  }
-  const int i = (int)(q % f.nx), j = (int)((q / f.nx) % f.ny), k = (int)(q / ((long long)f.nx * f.ny));
-  const double s0 = kOmegaC / (6.0 * f.inv_h2);   // omega / D_l
-  auto ev = [&](int a, int b, int cc) -> double {
-    if (a < 0 || b < 0 || cc < 0 || a >= f.nx || b >= f.ny || cc >= f.nz) return 0.0;
-    const long long fq = ((long long)cc * f.ny + b) * f.nx + a;
-    if (!f.mask[fq]) return 0.0;
-    const long long cq = ((long long)(cc >> 1) * c.ny + (b >> 1)) * c.nx + (a >> 1);
-    return s0 * f.b[fq] + c.e[cq];
-  };
-  const double e0 = ev(i, j, k);
-  const double ae = f.inv_h2 * (6.0 * e0 - ev(i - 1, j, k) - ev(i + 1, j, k) - ev(i, j - 1, k) - ev(i, j + 1, k) -
-                                ev(i, j, k - 1) - ev(i, j, k + 1));
-  f.e[q] = e0 + s0 * (f.b[q] - ae);
+  for (int d = 0; d < 8; ++d) s += dom_at(g, t0, 2 * I + (d & 1), 2 * J + ((d >> 1) & 1), 2 * K + (d >> 2));
+  c1.b[q] = 0.125 * s;   // zero on inactive cells: none of their children is in the domain
 }
 
 // z = post-smooth(c0 r + P e_1) with the domain's stencil; partials of r.z
-template <int R>
-__global__ __launch_bounds__(kB) void mg_up0(Fine g, const double* __restrict__ r, Cube c1, double* __restrict__ z,
-                                             double* __restrict__ prz, const double* sc) {
-  __shared__ double red[kB / 64];
+template <int R, int SX>
+__global__ __launch_bounds__(256) void mg_up0(ZMap zm, double c0, const double* __restrict__ r, Cube c1,
+                                              double* __restrict__ z, double* __restrict__ prz, const double* sc) {
+  __shared__ double tile[pcz::Tile<R, SX>::SIZE];
+  __shared__ double red[pcz::kZW];
   if (sc[S_DONE] != 0.0) return;
-  const long long p0 = (long long)blockIdx.x * g.chunk, p1 = min(p0 + g.chunk, g.n);
+  const int L = pcz::block_of(zm);
+  if (L >= zm.nwg) return;
   double s = 0.0;
-  for (long long p = p0 + threadIdx.x; p < p1; p += kB) {
-    int row, i, j, k;
-    int2 m;
-    row_of(g, p, row, m, i, j, k);
-    auto zv = [&](int a, int b, int cc) -> double {
-      if (a < 0 || b < 0 || cc < 0 || a >= g.nx || b >= g.ny || cc >= g.nz) return 0.0;
-      const int2 mm = g.meta[b + g.ny * cc];
-      const int lo = mm.y & 0xffff, hi = mm.y >> 16;
-      if (a < lo || a >= hi) return 0.0;
-      const long long cq = ((long long)(cc >> 1) * c1.ny + (b >> 1)) * c1.nx + (a >> 1);
-      return g.c0 * r[(long long)mm.x + a] + c1.e[cq];
-    };
-    const double zc = zv(i, j, k);
-    const double zn = zc + g.c0 * (r[p] - apply_A<R>(g, i, j, k, zc, zv));
-    z[p] = zn;
-    s += r[p] * zn;
-  }
+  const double* __restrict__ e1 = c1.e;
+  pcz::zmarch<R, SX>(
+      zm, L, tile,
+      [&](long long idx, int i, int j, int k) {
+        return c0 * r[idx] + e1[((long long)(k >> 1) * c1.ny + (j >> 1)) * c1.nx + (i >> 1)];
+      },
+      [&](long long idx, int, int, int, double qc, double acc) {
+        const double rv = r[idx];
+        const double zn = qc + c0 * (rv - acc);
+        z[idx] = zn;
+        s += rv * zn;
+      });
   const double t = pc_block_sum(s, red);
-  if (threadIdx.x == 0) prz[blockIdx.x] = t;
+  if (threadIdx.x == 0) prz[L] = t;
 }
 
-// PCG step t: convergence from r_t.r_t, beta = rz_t / rz_{t-1}, p = z + beta p_old,
-// Ap, partials of p.Ap.  Workgroup 0 records the scalars.
-template <int R>
-__global__ __launch_bounds__(kB) void mg_apply(Fine g, int t, int maxit, double tol, const double* __restrict__ z,
-                                               const double* __restrict__ p_old, double* __restrict__ p_new,
-                                               double* __restrict__ ap, const double* __restrict__ prr,
-                                               const double* __restrict__ prz, double* __restrict__ ppq, double* sc) {
-  __shared__ double red[kB / 64];
+// PCG step t: convergence from r_t.r_t, beta = rz_t / rz_{t-1}, p = z + beta p_old
+// (at t = 0 the caller passes p_old = z and beta is 0), A p, partials of p.Ap.
+// Workgroup 0 records the scalars.
+template <int R, int SX>
+__global__ __launch_bounds__(256) void mg_apply(ZMap zm, int t, int maxit, double tol, int nb_rr,
+                                                const double* __restrict__ z, const double* __restrict__ p_old,
+                                                double* __restrict__ p_new, double* __restrict__ ap,
+                                                const double* __restrict__ prr, const double* __restrict__ prz,
+                                                double* __restrict__ ppq, double* sc) {
+  __shared__ double tile[pcz::Tile<R, SX>::SIZE];
+  __shared__ double red[pcz::kZW];
   if (t > 0 && sc[S_DONE] != 0.0) return;
-  const double rr = sum_parts(prr, g.nb, red);
-  const double rz = sum_parts(prz, g.nb, red);
+  const int L = pcz::block_of(zm);
+  if (L >= zm.nwg) return;
+  const double rr = sum_parts(prr, nb_rr, red);
+  const double rz = sum_parts(prz, zm.nwg, red);
   const double tol1 = t == 0 ? tol * tol * rr : sc[S_TOL1];
   const bool done = t >= maxit || !(rr > tol1);
   const double beta = t > 0 ? rz / sc[S_RZ0 + ((t - 1) & 1)] : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (L == 0 && threadIdx.x == 0) {
     sc[S_RZ0 + (t & 1)] = rz;
     if (t == 0) {
       sc[S_TOL1] = tol1;
@@ -302,40 +207,30 @@ __global__ __launch_bounds__(kB) void mg_apply(Fine g, int t, int maxit, double 
     sc[S_DONE] = done ? 1.0 : 0.0;
   }
   if (done) return;
-  const long long p0 = (long long)blockIdx.x * g.chunk, p1 = min(p0 + g.chunk, g.n);
   double s = 0.0;
-  for (long long p = p0 + threadIdx.x; p < p1; p += kB) {
-    int row, i, j, k;
-    int2 m;
-    row_of(g, p, row, m, i, j, k);
-    auto pv = [&](int a, int b, int cc) -> double {
-      if (a < 0 || b < 0 || cc < 0 || a >= g.nx || b >= g.ny || cc >= g.nz) return 0.0;
-      const int2 mm = g.meta[b + g.ny * cc];
-      const int lo = mm.y & 0xffff, hi = mm.y >> 16;
-      if (a < lo || a >= hi) return 0.0;
-      const long long idx = (long long)mm.x + a;
-      return t == 0 ? z[idx] : z[idx] + beta * p_old[idx];
-    };
-    const double pc = pv(i, j, k);
-    const double apv = apply_A<R>(g, i, j, k, pc, pv);
-    p_new[p] = pc;
-    ap[p] = apv;
-    s += pc * apv;
-  }
+  pcz::zmarch<R, SX>(
+      zm, L, tile, [&](long long idx, int, int, int) { return z[idx] + beta * p_old[idx]; },
+      [&](long long idx, int, int, int, double qc, double acc) {
+        p_new[idx] = qc;
+        ap[idx] = acc;
+        s += qc * acc;
+      });
   const double tt = pc_block_sum(s, red);
-  if (threadIdx.x == 0) ppq[blockIdx.x] = tt;
+  if (threadIdx.x == 0) ppq[L] = tt;
 }
 
 // alpha = rz_t / p.Ap; x += alpha p; r -= alpha Ap; partials of r_{t+1}.r_{t+1}
-__global__ __launch_bounds__(kB) void mg_update(Fine g, int t, const double* sc, double* __restrict__ x,
+__global__ __launch_bounds__(kB) void mg_update(Fine g, int npq, int t, const double* sc, double* __restrict__ x,
                                                 double* __restrict__ r, const double* __restrict__ p,
                                                 const double* __restrict__ ap, const double* __restrict__ ppq,
                                                 double* __restrict__ prr) {
   __shared__ double red[kB / 64];
   if (sc[S_DONE] != 0.0) return;
-  const double pq = sum_parts(ppq, g.nb, red);
+  const int c = chunk_of(g.nb8);
+  if (c >= g.nb) return;
+  const double pq = sum_parts(ppq, npq, red);
   const double alpha = sc[S_RZ0 + (t & 1)] / pq;
-  const long long p0 = (long long)blockIdx.x * g.chunk, p1 = min(p0 + g.chunk, g.n);
+  const long long p0 = (long long)c * g.chunk, p1 = min(p0 + g.chunk, g.n);
   double s = 0.0;
   for (long long i = p0 + threadIdx.x; i < p1; i += kB) {
     x[i] += alpha * p[i];
@@ -344,23 +239,245 @@ __global__ __launch_bounds__(kB) void mg_update(Fine g, int t, const double* sc,
     s += rv * rv;
   }
   const double tt = pc_block_sum(s, red);
-  if (threadIdx.x == 0) prr[blockIdx.x] = tt;
+  if (threadIdx.x == 0) prr[c] = tt;
+}
+
+// ---------------------------------------------------------------- cube levels
+__device__ __forceinline__ double cube_at(const Cube& c, const double* __restrict__ a, int i, int j, int k) {
+  const bool in = i >= 0 && j >= 0 && k >= 0 && i < c.nx && j < c.ny && k < c.nz;
+  return in ? a[((long long)k * c.ny + j) * c.nx + i] : 0.0;
+}
+
+// b_{l+1} = R (b_l - A_l (omega b_l / D_l)) on the active cells.  One thread
+// per child (8 per coarse cell, adjacent lanes), summed across the 8 lanes.
+__global__ __launch_bounds__(kB) void mg_down(Cube f, Cube c, const double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  const long long gid = (long long)blockIdx.x * kB + threadIdx.x;
+  const long long q = gid >> 3;
+  const int d = (int)(gid & 7);
+  double v = 0.0;
+  if (q < c.cells) {
+    const int I = (int)(q % c.nx), J = (int)((q / c.nx) % c.ny), K = (int)(q / ((long long)c.nx * c.ny));
+    const int i = 2 * I + (d & 1), j = 2 * J + ((d >> 1) & 1), k = 2 * K + (d >> 2);
+    if (i < f.nx && j < f.ny && k < f.nz) {
+      const long long fq = ((long long)k * f.ny + j) * f.nx + i;
+      const double act = f.mask[fq] ? 1.0 : 0.0;
+      const double bc = f.b[fq];
+      const double nb = cube_at(f, f.b, i - 1, j, k) + cube_at(f, f.b, i + 1, j, k) + cube_at(f, f.b, i, j - 1, k) +
+                        cube_at(f, f.b, i, j + 1, k) + cube_at(f, f.b, i, j, k - 1) + cube_at(f, f.b, i, j, k + 1);
+      // t = b - (omega/6) (6 b - sum nb)
+      v = act * ((1.0 - kOmegaC) * bc + (kOmegaC / 6.0) * nb);
+    }
+  }
+  v += __shfl_xor(v, 1, 64);
+  v += __shfl_xor(v, 2, 64);
+  v += __shfl_xor(v, 4, 64);
+  if (d == 0 && q < c.cells) c.b[q] = 0.125 * v;
+}
+
+// e_l = post-smooth(omega b_l / D_l + P e_{l+1}) on the active cells
+__global__ __launch_bounds__(kB) void mg_up(Cube f, Cube c, const double* sc) {
+  if (sc[S_DONE] != 0.0) return;
+  const long long q = (long long)blockIdx.x * kB + threadIdx.x;
+  if (q >= f.cells) return;
+  const int i = (int)(q % f.nx), j = (int)((q / f.nx) % f.ny), k = (int)(q / ((long long)f.nx * f.ny));
+  const double s0 = kOmegaC / (6.0 * f.inv_h2);   // omega / D_l
+  auto ev = [&](int a, int b, int cc) -> double {
+    const bool in = a >= 0 && b >= 0 && cc >= 0 && a < f.nx && b < f.ny && cc < f.nz;
+    if (!in) return 0.0;
+    const long long fq = ((long long)cc * f.ny + b) * f.nx + a;
+    const long long cq = ((long long)(cc >> 1) * c.ny + (b >> 1)) * c.nx + (a >> 1);
+    const double act = f.mask[fq] ? 1.0 : 0.0;
+    return act * (s0 * f.b[fq] + c.e[cq]);
+  };
+  const double e0 = ev(i, j, k);
+  const double ae = f.inv_h2 * (6.0 * e0 - ev(i - 1, j, k) - ev(i + 1, j, k) - ev(i, j - 1, k) - ev(i, j + 1, k) -
+                                ev(i, j, k - 1) - ev(i, j, k + 1));
+  f.e[q] = f.mask[q] ? e0 + s0 * (f.b[q] - ae) : 0.0;
+}
+
+// ---------------------------------------------------------------- tail
+// Levels T..L (T >= 1) in LDS, one workgroup: restriction down to L, the
+// Chebyshev iteration on L, prolongation and post-smoothing back up to T.
+// Input b_T, output e_T (global).
+struct TailLev {
+  int nx, ny, nz, cells, off;   // off: first cell of the level in the LDS arrays
+  double inv_h2;
+  const unsigned char* mask;
+};
+struct Tail {
+  int nlev;                     // levels T..L = lev[0..nlev)
+  TailLev lev[kMaxLev];
+  const double* bT;
+  double* eT;
+  double lmin, lmax;
+};
+
+__global__ __launch_bounds__(kTailThreads) void mg_tail(Tail tl, const double* sc) {
+  __shared__ double bl[kTailCells], el[kTailCells];
+  __shared__ unsigned char ml[kTailCells];
+  __shared__ double xs[2][kCoarsest];
+  if (sc[S_DONE] != 0.0) return;
+  const int tid = threadIdx.x;
+  const int ncell = tl.lev[tl.nlev - 1].off + tl.lev[tl.nlev - 1].cells;
+  for (int l = 0; l < tl.nlev; ++l)
+    for (int q = tid; q < tl.lev[l].cells; q += kTailThreads) ml[tl.lev[l].off + q] = tl.lev[l].mask[q];
+  for (int q = tid; q < tl.lev[0].cells; q += kTailThreads) bl[q] = tl.bT[q];
+  for (int q = tid; q < ncell; q += kTailThreads) el[q] = 0.0;
+  __syncthreads();
+  auto at = [&](const double* a, const TailLev& v, int i, int j, int k) -> double {
+    const bool in = i >= 0 && j >= 0 && k >= 0 && i < v.nx && j < v.ny && k < v.nz;
+    return in ? a[v.off + (k * v.ny + j) * v.nx + i] : 0.0;
+  };
+  // down: b_{l+1} = R (b_l - A_l omega b_l / D_l)
+  for (int l = 0; l + 1 < tl.nlev; ++l) {
+    const TailLev& f = tl.lev[l];
+    const TailLev& c = tl.lev[l + 1];
+    for (int q = tid; q < c.cells; q += kTailThreads) {
+      const int I = q % c.nx, J = (q / c.nx) % c.ny, K = q / (c.nx * c.ny);
+      double s = 0.0;
+      for (int d = 0; d < 8; ++d) {
+        const int i = 2 * I + (d & 1), j = 2 * J + ((d >> 1) & 1), k = 2 * K + (d >> 2);
+        if (i >= f.nx || j >= f.ny || k >= f.nz) continue;
+        const int fq = f.off + (k * f.ny + j) * f.nx + i;
+        const double nb = at(bl, f, i - 1, j, k) + at(bl, f, i + 1, j, k) + at(bl, f, i, j - 1, k) +
+                          at(bl, f, i, j + 1, k) + at(bl, f, i, j, k - 1) + at(bl, f, i, j, k + 1);
+        s += ml[fq] ? (1.0 - kOmegaC) * bl[fq] + (kOmegaC / 6.0) * nb : 0.0;
+      }
+      bl[c.off + q] = 0.125 * s;
+    }
+    __syncthreads();
+  }
+  // coarsest: e = p(A) b, Chebyshev iteration on [lmin, lmax] from zero, one
+  // barrier per step (x alternates between two LDS buffers)
+  {
+    const TailLev& c = tl.lev[tl.nlev - 1];
+    const bool act = tid < c.cells && ml[c.off + tid];
+    const int I = tid % c.nx, J = (tid / c.nx) % c.ny, K = tid / (c.nx * c.ny);
+    const double theta = 0.5 * (tl.lmax + tl.lmin), delta = 0.5 * (tl.lmax - tl.lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    const double b = act ? bl[c.off + tid] : 0.0;
+    double x = 0.0, d = b / theta;
+    auto xat = [&](const double* a, int i, int j, int k) -> double {
+      const bool in = i >= 0 && j >= 0 && k >= 0 && i < c.nx && j < c.ny && k < c.nz;
+      return in ? a[(k * c.ny + j) * c.nx + i] : 0.0;
+    };
+    for (int it = 0; it < kCheb; ++it) {
+      x += d;
+      double* xb = xs[it & 1];
+      if (tid < c.cells) xb[tid] = act ? x : 0.0;
+      __syncthreads();
+      double r = 0.0;
+      if (act)
+        r = b - c.inv_h2 * (6.0 * x - xat(xb, I - 1, J, K) - xat(xb, I + 1, J, K) - xat(xb, I, J - 1, K) -
+                            xat(xb, I, J + 1, K) - xat(xb, I, J, K - 1) - xat(xb, I, J, K + 1));
+      const double rho1 = 1.0 / (2.0 * sigma - rho);
+      d = act ? rho1 * rho * d + 2.0 * rho1 / delta * r : 0.0;
+      rho = rho1;
+    }
+    if (tid < c.cells) el[c.off + tid] = act ? x + d : 0.0;
+    __syncthreads();
+  }
+  // up: e_l = post-smooth(omega b_l / D_l + P e_{l+1})
+  for (int l = tl.nlev - 2; l >= 0; --l) {
+    const TailLev& f = tl.lev[l];
+    const TailLev& c = tl.lev[l + 1];
+    const double s0 = kOmegaC / (6.0 * f.inv_h2);
+    auto ev = [&](int a, int b, int cc) -> double {
+      const bool in = a >= 0 && b >= 0 && cc >= 0 && a < f.nx && b < f.ny && cc < f.nz;
+      if (!in) return 0.0;
+      const int fq = f.off + (cc * f.ny + b) * f.nx + a;
+      return ml[fq] ? s0 * bl[fq] + el[c.off + ((cc >> 1) * c.ny + (b >> 1)) * c.nx + (a >> 1)] : 0.0;
+    };
+    for (int q = tid; q < f.cells; q += kTailThreads) {
+      const int i = q % f.nx, j = (q / f.nx) % f.ny, k = q / (f.nx * f.ny);
+      const double e0 = ev(i, j, k);
+      const double ae = f.inv_h2 * (6.0 * e0 - ev(i - 1, j, k) - ev(i + 1, j, k) - ev(i, j - 1, k) -
+                                    ev(i, j + 1, k) - ev(i, j, k - 1) - ev(i, j, k + 1));
+      const double e = ml[f.off + q] ? e0 + s0 * (bl[f.off + q] - ae) : 0.0;
+      if (l == 0) tl.eT[q] = e;
+      else el[f.off + q] = e;
+    }
+    __syncthreads();
+  }
+  if (tl.nlev == 1)
+    for (int q = tid; q < tl.lev[0].cells; q += kTailThreads) tl.eT[q] = el[q];
 }
 
 __global__ void mg_reset(double* sc) {
   if (threadIdx.x < S_COUNT) sc[threadIdx.x] = 0.0;
 }
 
+// ---------------------------------------------------------------- host
 struct MGCtx {
   void* grid;
   Fine g;
+  ZMap zm;
   int radius;
   int nlev;                   // cube levels 1..nlev
+  int tail;                   // first level held by mg_tail
   Cube lev[kMaxLev + 1];      // lev[1..nlev]
+  Tail tl;
   std::vector<void*> allocs;
-  double lmin, lmax;          // Chebyshev interval on the coarsest cube
   double* t0;
 };
+
+size_t work_bytes_for(long long n) {
+  return (size_t)(5 * n + 2 * kNB + 2 * pcz::kMaxWG + S_COUNT) * sizeof(double);
+}
+
+template <int R, int SX>
+struct FineLaunch {
+  static void res0(const MGCtx* c, hipStream_t st, const double* r, const double* sc) {
+    hipLaunchKernelGGL((mg_res0<R, SX>), dim3(8 * c->zm.nwg8), dim3(256), 0, st, c->zm, c->g.c0, r, c->t0, sc);
+  }
+  static void up0(const MGCtx* c, hipStream_t st, const double* r, double* z, double* prz, const double* sc) {
+    hipLaunchKernelGGL((mg_up0<R, SX>), dim3(8 * c->zm.nwg8), dim3(256), 0, st, c->zm, c->g.c0, r, c->lev[1], z, prz,
+                       sc);
+  }
+  static void apply(const MGCtx* c, hipStream_t st, int t, int maxit, double tol, const double* z,
+                    const double* p_old, double* p_new, double* ap, const double* prr, const double* prz, double* ppq,
+                    double* sc) {
+    hipLaunchKernelGGL((mg_apply<R, SX>), dim3(8 * c->zm.nwg8), dim3(256), 0, st, c->zm, t, maxit, tol, c->g.nb, z,
+                       p_old, p_new, ap, prr, prz, ppq, sc);
+  }
+  static void init(const MGCtx* c, hipStream_t st, const double* x, const double* rhs, double* r, double* prr) {
+    hipLaunchKernelGGL((mg_init<R>), dim3(8 * c->g.nb8), dim3(kB), 0, st, c->g, x, rhs, r, prr);
+  }
+};
+
+struct FineOps {
+  void (*res0)(const MGCtx*, hipStream_t, const double*, const double*);
+  void (*up0)(const MGCtx*, hipStream_t, const double*, double*, double*, const double*);
+  void (*apply)(const MGCtx*, hipStream_t, int, int, double, const double*, const double*, double*, double*,
+                const double*, const double*, double*, double*);
+  void (*init)(const MGCtx*, hipStream_t, const double*, const double*, double*, double*);
+};
+
+template <int R, int SX>
+FineOps fine_ops_rs() {
+  using F = FineLaunch<R, SX>;
+  return {F::res0, F::up0, F::apply, F::init};
+}
+
+template <int R>
+FineOps fine_ops_r(int sx) {
+  if (sx == 1) return fine_ops_rs<R, 1>();
+  if (sx == 2) return fine_ops_rs<R, 2>();
+  return fine_ops_rs<R, 4>();
+}
+
+bool fine_ops(int radius, int sx, FineOps& ops) {
+  switch (radius) {
+    case 1: ops = fine_ops_r<1>(sx); return true;
+    case 2: ops = fine_ops_r<2>(sx); return true;
+    case 3: ops = fine_ops_r<3>(sx); return true;
+    case 4: ops = fine_ops_r<4>(sx); return true;
+    case 6: ops = fine_ops_r<6>(sx); return true;
+    default: return false;
+  }
+}
 
 }  // namespace
 
@@ -386,6 +503,17 @@ int pc_mg_create(void* grid, double h, void** out_mg) {
   g.nb = (int)std::min<long long>(kNB, bp);
   g.chunk = ((bp + g.nb - 1) / g.nb) * kB;
   g.nb = (int)((g.n + g.chunk - 1) / g.chunk);
+  g.nb8 = (g.nb + 7) / 8;
+  int rz = 0;
+  if (pcz::zmap_init(grid, g.w, c->zm, &rz)) {
+    delete c;
+    PC_CHECK_ARG(false, "nx > 256");
+  }
+  FineOps ops;
+  if (!fine_ops(c->radius, c->zm.sx, ops)) {
+    delete c;
+    PC_CHECK_ARG(false, "unsupported stencil radius");
+  }
   // domain mask on the full cube (host)
   int nx, ny, nz;
   const int *lo_d, *hi_d, *off_d;
@@ -419,18 +547,34 @@ int pc_mg_create(void* grid, double h, void** out_mg) {
     PC_HIP(hipMalloc(&ed, L.cells * sizeof(double)));
     c->allocs.push_back(md); c->allocs.push_back(bd); c->allocs.push_back(ed);
     PC_HIP(hipMemcpy(md, m.data(), L.cells, hipMemcpyHostToDevice));
+    PC_HIP(hipMemset(bd, 0, L.cells * sizeof(double)));
+    PC_HIP(hipMemset(ed, 0, L.cells * sizeof(double)));
     L.mask = md; L.b = bd; L.e = ed;
     prev.swap(m);
     pn[0] = cn[0]; pn[1] = cn[1]; pn[2] = cn[2];
-    if (L.cells <= 512) break;
+    if (L.cells <= kCoarsest) break;
   }
-  PC_CHECK_ARG(c->lev[c->nlev].cells <= 512, "grid too large for the multigrid hierarchy");
+  PC_CHECK_ARG(c->lev[c->nlev].cells <= kCoarsest, "grid too large for the multigrid hierarchy");
+  // tail: the deepest run of levels whose cells fit the LDS arrays
+  c->tail = c->nlev;
+  long long tcells = c->lev[c->nlev].cells;
+  while (c->tail > 1 && tcells + c->lev[c->tail - 1].cells <= kTailCells) tcells += c->lev[--c->tail].cells;
+  Tail& tl = c->tl;
+  tl.nlev = c->nlev - c->tail + 1;
+  int off = 0;
+  for (int l = 0; l < tl.nlev; ++l) {
+    const Cube& L = c->lev[c->tail + l];
+    tl.lev[l] = {L.nx, L.ny, L.nz, (int)L.cells, off, L.inv_h2, L.mask};
+    off += (int)L.cells;
+  }
+  tl.bT = c->lev[c->tail].b;
+  tl.eT = c->lev[c->tail].e;
   // Chebyshev interval of the coarsest 7-point operator: lambda_max <= 12/h^2
   // (Gershgorin); lambda_min >= that of the full box (Dirichlet on a subset)
   const Cube& C = c->lev[c->nlev];
   auto lam1 = [](int n) { return 2.0 * (1.0 - std::cos(M_PI / (n + 1))); };
-  c->lmin = C.inv_h2 * (lam1(C.nx) + lam1(C.ny) + lam1(C.nz));
-  c->lmax = 12.0 * C.inv_h2;
+  tl.lmin = C.inv_h2 * (lam1(C.nx) + lam1(C.ny) + lam1(C.nz));
+  tl.lmax = 12.0 * C.inv_h2;
   PC_HIP(hipMalloc(&c->t0, g.n * sizeof(double)));
   c->allocs.push_back(c->t0);
   *out_mg = c;
@@ -440,6 +584,7 @@ int pc_mg_create(void* grid, double h, void** out_mg) {
 int pc_mg_destroy(void* mg) {
   MGCtx* c = static_cast<MGCtx*>(mg);
   if (!c) return pc::PC_OK;
+  (void)hipDeviceSynchronize();   // queued V-cycles may still read the level arrays
   for (void* p : c->allocs) (void)hipFree(p);
   delete c;
   return pc::PC_OK;
@@ -448,7 +593,7 @@ int pc_mg_destroy(void* mg) {
 int pc_mg_workspace(void* mg, size_t* out_bytes) {
   MGCtx* c = static_cast<MGCtx*>(mg);
   PC_CHECK_ARG(c && out_bytes, "null pointer");
-  *out_bytes = (size_t)(5 * c->g.n + 3 * kNB + 2 * kNB + S_COUNT) * sizeof(double);
+  *out_bytes = work_bytes_for(c->g.n);
   return pc::PC_OK;
 }
 
@@ -458,62 +603,38 @@ int pc_pcg_solve_mg(void* mg, const double* rhs, double* x, int maxit, double to
                     size_t work_bytes, int* its_out, double* res_out, void* stream) {
   MGCtx* c = static_cast<MGCtx*>(mg);
   PC_CHECK_ARG(c && rhs && x && work && its_out, "null pointer");
-  size_t need = 0;
-  pc_mg_workspace(mg, &need);
-  PC_CHECK_ARG(work_bytes >= need, "workspace too small");
+  PC_CHECK_ARG(work_bytes >= work_bytes_for(c->g.n), "workspace too small");
   PC_CHECK_ARG(((uintptr_t)work & 7) == 0, "workspace not 8-byte aligned");
   PC_CHECK_ARG(maxit >= 0, "maxit < 0");
   if (check_every < 1) check_every = 8;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  const Fine& g = c->g;
-  const long long n = g.n;
+  FineOps ops;
+  fine_ops(c->radius, c->zm.sx, ops);
+  const long long n = c->g.n;
   double* r = static_cast<double*>(work);
   double* z = r + n;
   double* p[2] = {z + n, z + 2 * n};
   double* ap = z + 3 * n;
   double* prr[2] = {ap + n, ap + n + kNB};
   double* prz = prr[1] + kNB;
-  double* ppq = prz + kNB;
-  double* sc = ppq + 2 * kNB;
-  const int nbp = (int)((n + kB - 1) / kB);
-  const int L = c->nlev;
+  double* ppq = prz + pcz::kMaxWG;
+  double* sc = ppq + pcz::kMaxWG;
+  const int T = c->tail;
   auto vcycle = [&](const double* rr, double* zz) {
-    switch (c->radius) {
-#define PC_MG_R(RR)                                                                                       \
-  case RR:                                                                                                \
-    hipLaunchKernelGGL(mg_res0<RR>, dim3(nbp), dim3(kB), 0, st, g, rr, c->t0, sc);                       \
-    break;
-      PC_MG_R(1) PC_MG_R(2) PC_MG_R(3) PC_MG_R(4) PC_MG_R(6)
-#undef PC_MG_R
-    }
-    hipLaunchKernelGGL(mg_restrict0, dim3((unsigned)((c->lev[1].cells + kB - 1) / kB)), dim3(kB), 0, st, g, c->t0,
+    ops.res0(c, st, rr, sc);
+    hipLaunchKernelGGL(mg_restrict0, dim3((unsigned)((c->lev[1].cells + kB - 1) / kB)), dim3(kB), 0, st, c->g, c->t0,
                        c->lev[1], sc);
-    for (int l = 1; l < L; ++l)
-      hipLaunchKernelGGL(mg_down, dim3((unsigned)((c->lev[l + 1].cells + kB - 1) / kB)), dim3(kB), 0, st,
+    for (int l = 1; l < T; ++l)
+      hipLaunchKernelGGL(mg_down, dim3((unsigned)((8 * c->lev[l + 1].cells + kB - 1) / kB)), dim3(kB), 0, st,
                          c->lev[l], c->lev[l + 1], sc);
-    hipLaunchKernelGGL(mg_coarsest, dim3(1), dim3(512), 0, st, c->lev[L], c->lmin, c->lmax, sc);
-    for (int l = L - 1; l >= 1; --l)
+    hipLaunchKernelGGL(mg_tail, dim3(1), dim3(kTailThreads), 0, st, c->tl, sc);
+    for (int l = T - 1; l >= 1; --l)
       hipLaunchKernelGGL(mg_up, dim3((unsigned)((c->lev[l].cells + kB - 1) / kB)), dim3(kB), 0, st, c->lev[l],
                          c->lev[l + 1], sc);
-    switch (c->radius) {
-#define PC_MG_R(RR)                                                                                        \
-  case RR:                                                                                                 \
-    hipLaunchKernelGGL(mg_up0<RR>, dim3(g.nb), dim3(kB), 0, st, g, rr, c->lev[1], zz, prz, sc);          \
-    break;
-      PC_MG_R(1) PC_MG_R(2) PC_MG_R(3) PC_MG_R(4) PC_MG_R(6)
-#undef PC_MG_R
-    }
+    ops.up0(c, st, rr, zz, prz, sc);
   };
   hipLaunchKernelGGL(mg_reset, dim3(1), dim3(64), 0, st, sc);
-  switch (c->radius) {
-#define PC_MG_R(RR)                                                                                            \
-  case RR:                                                                                                     \
-    hipLaunchKernelGGL(mg_init<RR>, dim3(g.nb), dim3(kB), 0, st, g, x, rhs, r, prr[0]);                       \
-    break;
-    PC_MG_R(1) PC_MG_R(2) PC_MG_R(3) PC_MG_R(4) PC_MG_R(6)
-#undef PC_MG_R
-    default: PC_CHECK_ARG(false, "unsupported stencil radius");
-  }
+  ops.init(c, st, x, rhs, r, prr[0]);
   vcycle(r, z);
   PC_LAUNCH_CHECK();
   static thread_local double* host = nullptr;
@@ -521,18 +642,12 @@ int pc_pcg_solve_mg(void* mg, const double* rhs, double* x, int maxit, double to
   int status = pc::PC_OK;
   for (int t0 = 0; t0 <= maxit; t0 += check_every) {
     for (int t = t0; t < t0 + check_every && t <= maxit; ++t) {
-      switch (c->radius) {
-#define PC_MG_R(RR)                                                                                          \
-  case RR:                                                                                                   \
-    hipLaunchKernelGGL(mg_apply<RR>, dim3(g.nb), dim3(kB), 0, st, g, t, maxit, tol, z, p[t & 1],             \
-                       p[(t + 1) & 1], ap, prr[t & 1], prz, ppq, sc);                                        \
-    break;
-        PC_MG_R(1) PC_MG_R(2) PC_MG_R(3) PC_MG_R(4) PC_MG_R(6)
-#undef PC_MG_R
-      }
+      // at t = 0 the direction is z itself: p_old = z with beta = 0
+      const double* p_old = t == 0 ? z : p[t & 1];
+      ops.apply(c, st, t, maxit, tol, z, p_old, p[(t + 1) & 1], ap, prr[t & 1], prz, ppq, sc);
       if (t < maxit) {
-        hipLaunchKernelGGL(mg_update, dim3(g.nb), dim3(kB), 0, st, g, t, sc, x, r, p[(t + 1) & 1], ap, ppq,
-                           prr[(t + 1) & 1]);
+        hipLaunchKernelGGL(mg_update, dim3(8 * c->g.nb8), dim3(kB), 0, st, c->g, c->zm.nwg, t, sc, x, r,
+                           p[(t + 1) & 1], ap, ppq, prr[(t + 1) & 1]);
         vcycle(r, z);
       }
       if (hipGetLastError() != hipSuccess) {

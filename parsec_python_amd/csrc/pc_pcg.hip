@@ -20,18 +20,16 @@
 // maxit) both kernels return at once, so the host only reads the state back
 // every `check_every` iterations.
 #include "pc_common.h"
+#include "pc_zmarch.h"
 #include <algorithm>
 
-int pc_internal_grid_points(void* grid, int* ny, int* nz, int* radius, const int2** meta, const int** prow,
-                            double* w8);
-int pc_internal_grid_zmap(void* grid, int* nx, const int** zrange);
 long long pc_internal_grid_npts(void* grid);
 
 namespace {
 
 constexpr int kBlk = 512;       // threads per workgroup
 constexpr int kMaxNB = 512;     // workgroups of the point-wise launches (= r.r partials)
-constexpr int kMaxWG = 2048;    // workgroups of the z-march step (= p.Ap partials)
+constexpr int kMaxWG = pcz::kMaxWG;   // workgroups of the z-march step (= p.Ap partials)
 // scalars for the host (written by workgroup 0); r.r of iteration t in slot S_RO0 + (t & 1)
 enum { S_RO0 = 0, S_RO1, S_TOL1, S_DONE, S_ITS, S_RES, S_COUNT = 8 };
 enum Mode { M_FIRST = 1, M_ALIAS = 2, M_NORMAL = 3 };
@@ -127,35 +125,10 @@ __global__ __launch_bounds__(kBlk) void cg_init(PointMap g, const double* __rest
 // ---- iteration step t (t iterations completed so far) ------------------------
 // rr partials of r_t.r_t -> convergence decision (t >= maxit or r.r <= tol^2
 // r0.r0) and beta = r_t.r_t / r_{t-1}.r_{t-1} in every workgroup; workgroup 0
-// records the scalars.  Then p_new = r + beta p_old, A p_new and the partials of
-// p_new . A p_new, as a z-march:
-// a workgroup owns 4 y-rows (one per wavefront) and a chunk of zc z-planes of
-// them, and marches z: each lane keeps its points' 2R+1 z-values of p_new in
-// registers, the current plane of the 4 rows plus R halo rows on each side
-// sits in LDS (two tiles, one barrier per plane) for the x/y differences, and
-// the loads for plane z+1 are issued before plane z is computed.  p_new is
-// formed on the fly from r and p_old.  Partials: one per workgroup.
-constexpr int kZW = 4;
-
-struct ZMap {
-  const int2* meta;
-  const int* zrange;   // [2*ny]
-  int nx, ny, nz;
-  int nslab, nzc, zc;  // 4-row slabs, z-chunks per slab, planes per chunk
-  int nwg, nwg8;       // nslab * nzc workgroups, ceil(nwg / 8)
-  double w[8];
-};
-
-__device__ __forceinline__ int2 zrow(const ZMap& g, int j, int z) {
-  return (j >= 0 && j < g.ny && z >= 0 && z < g.nz) ? g.meta[j + g.ny * z] : make_int2(0, 0);
-}
-
-template <int MODE>
-__device__ __forceinline__ double qload(const double* __restrict__ a, const double* __restrict__ b, double beta,
-                                        int2 m, int i) {
-  const int lo = m.y & 0xffff, hi = m.y >> 16;
-  return (i >= lo && i < hi) ? qv<MODE>(a, b, beta, (long long)m.x + i) : 0.0;
-}
+// records the scalars.  Then p_new = r + beta p_old (formed on the fly at every
+// stencil point of the z-march), A p_new and the partials of p_new . A p_new,
+// one per workgroup.
+using pcz::ZMap;
 
 template <int R, int SX, int MODE>
 __global__ __launch_bounds__(256) void cg_apply_z(ZMap g, int t, int maxit, double tol, int nb_rr,
@@ -163,14 +136,10 @@ __global__ __launch_bounds__(256) void cg_apply_z(ZMap g, int t, int maxit, doub
                                                   double* __restrict__ p_new, double* __restrict__ ap,
                                                   const double* __restrict__ part_rr, double* __restrict__ part_pq,
                                                   double* sc) {
-  constexpr int TY = kZW;
-  constexpr int LW = 64 * SX + 2 * R;
-  constexpr int LR = TY + 2 * R;
-  constexpr int HT = (2 * R + kZW - 1) / kZW;
-  __shared__ double tile[2][LR * LW];
-  __shared__ double red[kZW];
+  __shared__ double tile[pcz::Tile<R, SX>::SIZE];
+  __shared__ double red[pcz::kZW];
   if (t > 0 && sc[S_DONE] != 0.0) return;
-  const int L = (blockIdx.x & 7) * g.nwg8 + (blockIdx.x >> 3);
+  const int L = pcz::block_of(g);
   if (L >= g.nwg) return;
   const double rr = sum_parts(part_rr, nb_rr, red);
   const double tol1 = t == 0 ? tol * tol * rr : sc[S_TOL1];
@@ -184,103 +153,14 @@ __global__ __launch_bounds__(256) void cg_apply_z(ZMap g, int t, int maxit, doub
     sc[S_DONE] = done ? 1.0 : 0.0;
   }
   if (done) return;
-  const int slab = L % g.nslab;
-  const int zci = L / g.nslab;
-  const int j0 = slab * TY;
-  int zb = g.nz, ze = 0;
-  for (int q = 0; q < TY; ++q) {
-    const int j = j0 + q;
-    if (j < g.ny) {
-      zb = min(zb, g.zrange[2 * j]);
-      ze = max(ze, g.zrange[2 * j + 1]);
-    }
-  }
-  const int z0 = zb + zci * g.zc;
-  const int z1 = min(ze, z0 + g.zc);
   double sacc = 0.0;
-  if (z0 < z1) {
-    const int lane = threadIdx.x & 63;
-    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int j = j0 + wv;
-    for (int e = threadIdx.x; e < 2 * LR * LW; e += 256) (&tile[0][0])[e] = 0.0;
-    __syncthreads();
-    double q[SX][2 * R + 1];
-    double nq[SX], hc[HT][SX], hn[HT][SX];
-    int2 mcur, mnext;
-#pragma unroll
-    for (int s = 0; s < SX; ++s) {
-      q[s][0] = 0.0;
-#pragma unroll
-      for (int h = 0; h < HT; ++h) hc[h][s] = hn[h][s] = 0.0;
-    }
-#pragma unroll
-    for (int dz = 1; dz <= 2 * R; ++dz) {
-      const int2 m = zrow(g, j, z0 - R - 1 + dz);
-#pragma unroll
-      for (int s = 0; s < SX; ++s) q[s][dz] = qload<MODE>(r, p_old, beta, m, lane + 64 * s);
-    }
-    auto prefetch = [&](int z) {
-      const int2 mq = zrow(g, j, z + R);
-      mnext = zrow(g, j, z);
-#pragma unroll
-      for (int s = 0; s < SX; ++s) nq[s] = qload<MODE>(r, p_old, beta, mq, lane + 64 * s);
-#pragma unroll
-      for (int h = 0; h < HT; ++h) {
-        const int hr = wv + kZW * h;
-        if (hr < 2 * R) {
-          const int tr = hr < R ? hr : hr + TY;
-          const int2 mh = zrow(g, j0 - R + tr, z);
-#pragma unroll
-          for (int s = 0; s < SX; ++s) hn[h][s] = qload<MODE>(r, p_old, beta, mh, lane + 64 * s);
-        }
-      }
-    };
-    prefetch(z0);
-    for (int z = z0; z < z1; ++z) {
-      double* tb = tile[(z - z0) & 1];
-      mcur = mnext;
-#pragma unroll
-      for (int s = 0; s < SX; ++s) {
-#pragma unroll
-        for (int d = 0; d < 2 * R; ++d) q[s][d] = q[s][d + 1];
-        q[s][2 * R] = nq[s];
-#pragma unroll
-        for (int h = 0; h < HT; ++h) hc[h][s] = hn[h][s];
-      }
-      if (z + 1 < z1) prefetch(z + 1);
-#pragma unroll
-      for (int s = 0; s < SX; ++s) tb[(wv + R) * LW + R + lane + 64 * s] = q[s][R];
-#pragma unroll
-      for (int h = 0; h < HT; ++h) {
-        const int hr = wv + kZW * h;
-        if (hr < 2 * R) {
-          const int tr = hr < R ? hr : hr + TY;
-#pragma unroll
-          for (int s = 0; s < SX; ++s) tb[tr * LW + R + lane + 64 * s] = hc[h][s];
-        }
-      }
-      __builtin_amdgcn_s_waitcnt(0xc07f);   // LDS only: the prefetch stays in flight
-      __builtin_amdgcn_s_barrier();
-      const int lo = mcur.y & 0xffff, hi = mcur.y >> 16;
-#pragma unroll
-      for (int s = 0; s < SX; ++s) {
-        const int i = lane + 64 * s;
-        const double* trow = tb + (wv + R) * LW + R + i;
-        double acc = g.w[0] * q[s][R];
-#pragma unroll
-        for (int d = 1; d <= R; ++d) {
-          const double nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[s][R - d] + q[s][R + d];
-          acc += g.w[d] * nb;
-        }
-        if (i >= lo && i < hi) {
-          const long long idx = (long long)mcur.x + i;
-          p_new[idx] = q[s][R];
-          ap[idx] = acc;
-          sacc += q[s][R] * acc;
-        }
-      }
-    }
-  }
+  pcz::zmarch<R, SX>(
+      g, L, tile, [&](long long idx, int, int, int) { return qv<MODE>(r, p_old, beta, idx); },
+      [&](long long idx, int, int, int, double qc, double acc) {
+        p_new[idx] = qc;
+        ap[idx] = acc;
+        sacc += qc * acc;
+      });
   const double bs = pc_block_sum(sacc, red);
   if (threadIdx.x == 0) part_pq[L] = bs;
 }
@@ -397,19 +277,10 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
   for (int d = 0; d < 8; ++d) g.w[d] = w8[d] * inv_h2;
   g.w[0] *= 3.0;
   ZMap zm;
-  zm.meta = g.meta;
-  zm.ny = g.ny;
-  zm.nz = g.nz;
-  pc_internal_grid_zmap(grid, &zm.nx, &zm.zrange);
-  for (int d = 0; d < 8; ++d) zm.w[d] = g.w[d];
-  zm.nslab = (zm.ny + kZW - 1) / kZW;
-  zm.zc = 6;   // measured on Si87H76: 38 us per iteration at 6 planes, 41 us at 2
-  while ((long long)zm.nslab * ((zm.nz + zm.zc - 1) / zm.zc) > kMaxWG) ++zm.zc;
-  zm.nzc = (zm.nz + zm.zc - 1) / zm.zc;
-  zm.nwg = zm.nslab * zm.nzc;
-  zm.nwg8 = (zm.nwg + 7) / 8;
-  const int sx = zm.nx <= 64 ? 1 : (zm.nx <= 128 ? 2 : 4);
-  PC_CHECK_ARG(zm.nx <= 256, "nx > 256");
+  int radius_z = 0;
+  const int zbad = pcz::zmap_init(grid, g.w, zm, &radius_z);
+  const int sx = zm.sx;
+  PC_CHECK_ARG(!zbad, "nx > 256");
   Launchers L;
   switch (radius) {
     case 1: L = launchers<1>(sx); break;

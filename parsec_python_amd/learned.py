@@ -366,8 +366,9 @@ def learned_start(prob, net: UNet, cg_maxit: int = 1000, cg_tol: float = 1e-10):
     h3 = prob.h ** 3
     rhs = (4.0 * math.pi / h3) * (rho1 - prob.rho0)
     dv = torch.zeros_like(rhs)
-    hs = HartreeSolver(prob.H.dgrid, prob.h, reference_aliasing=False)
+    hs = HartreeSolver(prob.H.dgrid, prob.h, reference_aliasing=False, prec=getattr(prob.cfg, "pcg_prec", "none"))
     hs.solve(rhs, dv, cg_maxit, cg_tol)
+    hs.close()
     return rho1, prob.hpot0 + dv
 
 
