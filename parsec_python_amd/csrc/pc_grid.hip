@@ -22,6 +22,7 @@
 // in LDS for the x/y differences.  Workgroup ids are remapped so the slabs of
 // one vector run on the same XCD (its L2 then serves the halo rows).
 #include "pc_common.h"
+#include <type_traits>
 #include <vector>
 #include <algorithm>
 
@@ -74,11 +75,12 @@ __device__ __forceinline__ T meta_val(const T* __restrict__ y, int2 m, int i) {
 }
 
 // Software-pipelined z-march: while plane z is computed from LDS and the
-// register queues, the loads for plane z+1 (queue entry z+1+R, halo rows,
-// V and x) are already in flight.  A workgroup handles NV vectors at once:
-// the row metadata, the V values and the control flow are shared by them.
-// Two LDS tiles alternate, so one barrier per plane suffices.
-template <typename T, int R, int SX, int TYW, int NV, int NB, bool HASX, bool DOT>
+// register queues, the loads for the next PD planes (queue entry z+R+1.., halo
+// rows, V and x) are already in flight (PD register stages, the z loop unrolled
+// by PD).  A workgroup handles NV vectors at once: the row metadata, the V
+// values and the control flow are shared by them.  Two LDS tiles alternate,
+// so one barrier per plane suffices.
+template <typename T, int R, int SX, int TYW, int NV, int NB, bool HASX, bool DOT, int PD = 1>
 __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
                                                       T* Out, const T* __restrict__ V, T ca, T cb, T cc,
                                                       double* __restrict__ dot_part) {
@@ -88,6 +90,8 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   constexpr int LW = NXP + 2 * R;   // LDS row stride (R zero columns each side)
   constexpr int LR = TY + 2 * R;    // LDS rows
   constexpr int HT = (2 * R + NW - 1) / NW;  // halo rows per wave
+  static_assert(PD == 1 || PD == 2, "prefetch depth 1 or 2");
+  static_assert(PD == 1 || NB == 2, "two planes in flight need two LDS tiles");
   __shared__ T tile[NB][NV][LR * LW];
   __shared__ double red[NW];
 
@@ -139,12 +143,43 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   auto rowmeta = [&](int j, int z) -> int2 {
     return (j >= 0 && j < ny && z >= 0 && z < nz) ? meta[j + ny * z] : make_int2(0, 0);
   };
+  // Row metadata windows: lane l of a window holds meta(j, wb + l), so the
+  // per-plane lookups are register reads (readlane) and the addresses of a
+  // plane's loads never wait for a metadata load; a window is reloaded once
+  // every 64 planes.  Own row: planes z (output) and z + R (queue front).
+  struct Win {
+    int2 v;
+    int wb;
+  };
+  auto win_load = [&](Win& w, int j, int wb) {
+    w.wb = wb;
+    w.v = rowmeta(j, wb + lane);
+  };
+  auto win_at = [&](const Win& w, int z) -> int2 {
+    const int l = z - w.wb;
+    return make_int2(__builtin_amdgcn_readlane(w.v.x, l), __builtin_amdgcn_readlane(w.v.y, l));
+  };
+  static_assert(TYW == 1, "metadata windows hold one own row per wave");
+  Win wown, whalo[HT];
+  const int jown = j0 + wv;
+  win_load(wown, jown, zb - R);
+#pragma unroll
+  for (int t = 0; t < HT; ++t) {
+    const int hr = wv + NW * t;
+    const int tr = hr < R ? hr : hr + TY;
+    win_load(whalo[t], j0 - R + tr, zb);
+  }
 
   T q[NV][TYW][SX][2 * R + 1];
-  T nq[NV][TYW][SX], xq[NV][TYW][SX], nxv[NV][TYW][SX];
-  T vq[TYW][SX], nv[TYW][SX];
-  T hc[NV][HT][SX] = {}, hn[NV][HT][SX] = {};
-  int2 mcur[TYW], mnext[TYW];
+  T xq[NV][TYW][SX];
+  T vq[TYW][SX];
+  T hc[NV][HT][SX] = {};
+  int2 mcur[TYW];
+  // prefetch stages
+  T nq[PD][NV][TYW][SX], nxv[PD][NV][TYW][SX];
+  T nv[PD][TYW][SX];
+  T hn[PD][NV][HT][SX] = {};
+  int2 mnext[PD][TYW];
 
   // prefill planes zb-R .. zb+R-1
 #pragma unroll
@@ -156,31 +191,32 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
       for (int s = 0; s < SX; ++s) q[u][r][s][0] = T(0);
 #pragma unroll
     for (int dz = 1; dz <= 2 * R; ++dz) {
-      const int2 m = rowmeta(j, zb - R - 1 + dz);
+      const int2 m = win_at(wown, zb - R - 1 + dz);
 #pragma unroll
       for (int u = 0; u < NV; ++u)
 #pragma unroll
         for (int s = 0; s < SX; ++s) q[u][r][s][dz] = meta_val(yv[u], m, lane + 64 * s);
     }
   }
-  // prefetch everything plane z needs, for z = zb
-  auto prefetch = [&](int z) {
+  // issue everything plane z needs into stage S
+  auto prefetch = [&](auto stage, int z) {
+    constexpr int S = decltype(stage)::value;
+    if (z + R >= wown.wb + 64) win_load(wown, jown, z);
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
-      const int j = j0 + wv + NW * r;
-      const int2 mq = rowmeta(j, z + R);
-      const int2 mo = rowmeta(j, z);
-      mnext[r] = mo;
+      const int2 mq = win_at(wown, z + R);
+      const int2 mo = win_at(wown, z);
+      mnext[S][r] = mo;
       const int lo = mo.y & 0xffff, hi = mo.y >> 16;
 #pragma unroll
       for (int s = 0; s < SX; ++s) {
         const int i = lane + 64 * s;
         const bool in = i >= lo && i < hi;
-        nv[r][s] = in ? V[mo.x + i] : T(0);
+        nv[S][r][s] = in ? V[mo.x + i] : T(0);
 #pragma unroll
         for (int u = 0; u < NV; ++u) {
-          nq[u][r][s] = meta_val(yv[u], mq, i);
-          nxv[u][r][s] = (in && hasx) ? xv[u][mo.x + i] : T(0);
+          nq[S][u][r][s] = meta_val(yv[u], mq, i);
+          nxv[S][u][r][s] = (in && hasx) ? xv[u][mo.x + i] : T(0);
         }
       }
     }
@@ -189,31 +225,31 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
       const int hr = wv + NW * t;
       if (hr < 2 * R) {
         const int tr = hr < R ? hr : hr + TY;
-        const int2 mh = rowmeta(j0 - R + tr, z);
+        if (z >= whalo[t].wb + 64) win_load(whalo[t], j0 - R + tr, z);
+        const int2 mh = win_at(whalo[t], z);
 #pragma unroll
         for (int u = 0; u < NV; ++u)
 #pragma unroll
-          for (int s = 0; s < SX; ++s) hn[u][t][s] = meta_val(yv[u], mh, lane + 64 * s);
+          for (int s = 0; s < SX; ++s) hn[S][u][t][s] = meta_val(yv[u], mh, lane + 64 * s);
       }
     }
   };
-  prefetch(zb);
-
-  for (int z = zb; z < ze; ++z) {
+  // plane z from stage S: commit, refill the stage with plane z+PD, compute
+  auto plane = [&](auto stage, int z) {
+    constexpr int S = decltype(stage)::value;
     const int b = NB == 2 ? ((z - zb) & 1) : 0;
-    // commit the prefetched plane
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
-      mcur[r] = mnext[r];
+      mcur[r] = mnext[S][r];
 #pragma unroll
       for (int s = 0; s < SX; ++s) {
-        vq[r][s] = nv[r][s];
+        vq[r][s] = nv[S][r][s];
 #pragma unroll
         for (int u = 0; u < NV; ++u) {
 #pragma unroll
           for (int d = 0; d < 2 * R; ++d) q[u][r][s][d] = q[u][r][s][d + 1];
-          q[u][r][s][2 * R] = nq[u][r][s];
-          xq[u][r][s] = nxv[u][r][s];
+          q[u][r][s][2 * R] = nq[S][u][r][s];
+          xq[u][r][s] = nxv[S][u][r][s];
         }
       }
     }
@@ -222,9 +258,9 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
 #pragma unroll
       for (int t = 0; t < HT; ++t)
 #pragma unroll
-        for (int s = 0; s < SX; ++s) hc[u][t][s] = hn[u][t][s];
-    // issue the loads for the next plane
-    if (z + 1 < ze) prefetch(z + 1);
+        for (int s = 0; s < SX; ++s) hc[u][t][s] = hn[S][u][t][s];
+    // issue the loads for plane z + PD
+    if (z + PD < ze) prefetch(stage, z + PD);
     // plane z into LDS
 #pragma unroll
     for (int u = 0; u < NV; ++u) {
@@ -279,6 +315,18 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
       }
     }
     if (NB == 1) lds_barrier();   // one tile: the next plane overwrites it
+  };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, PD - 1>;
+  prefetch(S0{}, zb);
+  if constexpr (PD == 2) {
+    if (zb + 1 < ze) prefetch(S1{}, zb + 1);
+    for (int z = zb; z < ze; z += 2) {
+      plane(S0{}, z);
+      if (z + 1 < ze) plane(S1{}, z + 1);
+    }
+  } else {
+    for (int z = zb; z < ze; ++z) plane(S0{}, z);
   }
   if constexpr (DOT) {
     const double s = pc_block_sum(dot, red);
@@ -291,12 +339,12 @@ constexpr int zmarch_lds_bytes() {
   return NB * NV * (4 * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
 }
 
-template <typename T, int R, int SX, int TYW, int NV, int NB = 2>
+template <typename T, int R, int SX, int TYW, int NV, int NB = 2, int PD = 1>
 int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                double ca, double cb, double cc, double* dot_part, hipStream_t st) {
   if constexpr (NV > 1 && zmarch_lds_bytes<T, R, SX, TYW, NV, NB>() > 112 * 1024) {
     // too much LDS for several vectors per workgroup: one vector per workgroup
-    return launch_cfg<T, R, SX, TYW, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    return launch_cfg<T, R, SX, TYW, 1, NB, PD>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
   } else {
   constexpr int TY = 4 * TYW;
   StArgs a;
@@ -314,20 +362,20 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
   if (dot_part) {
     if constexpr (NV == 1) {
       if (X)
-        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, true>), grid, dim3(256), 0, st, a, Y, X, Out,
+        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, true, PD>), grid, dim3(256), 0, st, a, Y, X, Out,
                            V, T(ca), T(cb), T(cc), dot_part);
       else
-        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, true>), grid, dim3(256), 0, st, a, Y, X,
+        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, true, PD>), grid, dim3(256), 0, st, a, Y, X,
                            Out, V, T(ca), T(cb), T(cc), dot_part);
     } else {
       pc::set_error("stencil: partial sums need one vector per workgroup");
       return pc::PC_ERR_ARG;
     }
   } else if (X) {
-    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, false>), grid, dim3(256), 0, st, a, Y, X, Out, V,
+    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, false, PD>), grid, dim3(256), 0, st, a, Y, X, Out, V,
                        T(ca), T(cb), T(cc), dot_part);
   } else {
-    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, false>), grid, dim3(256), 0, st, a, Y, X, Out,
+    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, false, PD>), grid, dim3(256), 0, st, a, Y, X, Out,
                        V, T(ca), T(cb), T(cc), dot_part);
   }
   PC_LAUNCH_CHECK();
@@ -335,21 +383,20 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
   }
 }
 
-// One y-row per wavefront (TYW = 1) and as many vectors per workgroup as keep
-// the two LDS tiles within 56 KB: 4 FP32 vectors at radius 4, 2 FP64 vectors,
-// 2 FP32 vectors at radius 6.  Measured on the Si87H76 and 128^3 workloads
-// against taller slabs, 1..8 vectors and a single LDS tile (NB = 1, one more
-// barrier per plane): FP32 Si87H76 1.17 ms here, 1.13 ms for 2 vectors with
-// one tile, 1.37-2.3 ms for the others.
+// One y-row per wavefront (TYW = 1), two vectors per workgroup where the two
+// LDS tiles fit in 56 KB (FP32 and FP64 up to radius 6 at nx <= 128), else one.
+// Measured on Si87H76 (one recurrence step, 275 vectors) with the row metadata
+// in register windows: FP32 0.95 ms with 2 vectors, 0.96 ms with 1, 1.04 ms
+// with 4; FP64 1.72 ms with 2, 1.79 ms with 1.  A prefetch depth of two planes
+// (PD = 2) was slower (1.22 ms FP32, 4 vectors): the registers it takes cost
+// more occupancy than the extra plane in flight gains.
 template <typename T, int R, int SX>
 int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                 double ca, double cb, double cc, double* dot_part, hipStream_t st) {
   constexpr int kLds = 56 * 1024;
   if (dot_part)   // per-vector partial sums need one vector per workgroup
     return launch_cfg<T, R, SX, 1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-  if constexpr (zmarch_lds_bytes<T, R, SX, 1, 4>() <= kLds)
-    return launch_cfg<T, R, SX, 1, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-  else if constexpr (zmarch_lds_bytes<T, R, SX, 1, 2>() <= kLds)
+  if constexpr (zmarch_lds_bytes<T, R, SX, 1, 2>() <= kLds)
     return launch_cfg<T, R, SX, 1, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
   else
     return launch_cfg<T, R, SX, 1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);

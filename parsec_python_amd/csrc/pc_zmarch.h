@@ -101,6 +101,30 @@ __device__ __forceinline__ void zmarch(const ZMap& g, int L, double* tile, Src s
     const int lo = m.y & 0xffff, hi = m.y >> 16;
     return (i >= lo && i < hi) ? src((long long)m.x + i, i, jj, z) : 0.0;
   };
+  // Row metadata windows: lane l holds meta(row, wb + l), so a plane's
+  // lookups are register reads and its loads never wait for a metadata load;
+  // a window is reloaded once every 64 planes.  The own row serves planes z
+  // (output) and z + R (queue front).
+  struct Win {
+    int2 v;
+    int wb;
+  };
+  auto win_load = [&](Win& w, int jj, int wb) {
+    w.wb = wb;
+    w.v = zrow(g, jj, wb + lane);
+  };
+  auto win_at = [&](const Win& w, int z) -> int2 {
+    const int l = z - w.wb;
+    return make_int2(__builtin_amdgcn_readlane(w.v.x, l), __builtin_amdgcn_readlane(w.v.y, l));
+  };
+  Win wown, whalo[HT];
+  win_load(wown, j, z0 - R);
+#pragma unroll
+  for (int h = 0; h < HT; ++h) {
+    const int hr = wv + kZW * h;
+    const int tr = hr < R ? hr : hr + TY;
+    win_load(whalo[h], j0 - R + tr, z0);
+  }
   for (int e = threadIdx.x; e < 2 * LR * LW; e += 256) tile[e] = 0.0;
   __syncthreads();
   double q[SX][2 * R + 1];
@@ -115,13 +139,14 @@ __device__ __forceinline__ void zmarch(const ZMap& g, int L, double* tile, Src s
 #pragma unroll
   for (int dz = 1; dz <= 2 * R; ++dz) {
     const int zz = z0 - R - 1 + dz;
-    const int2 m = zrow(g, j, zz);
+    const int2 m = win_at(wown, zz);
 #pragma unroll
     for (int s = 0; s < SX; ++s) q[s][dz] = load(m, lane + 64 * s, j, zz);
   }
   auto prefetch = [&](int z) {
-    const int2 mq = zrow(g, j, z + R);
-    mnext = zrow(g, j, z);
+    if (z + R >= wown.wb + 64) win_load(wown, j, z);
+    const int2 mq = win_at(wown, z + R);
+    mnext = win_at(wown, z);
 #pragma unroll
     for (int s = 0; s < SX; ++s) nq[s] = load(mq, lane + 64 * s, j, z + R);
 #pragma unroll
@@ -130,7 +155,8 @@ __device__ __forceinline__ void zmarch(const ZMap& g, int L, double* tile, Src s
       if (hr < 2 * R) {
         const int tr = hr < R ? hr : hr + TY;
         const int jh = j0 - R + tr;
-        const int2 mh = zrow(g, jh, z);
+        if (z >= whalo[h].wb + 64) win_load(whalo[h], jh, z);
+        const int2 mh = win_at(whalo[h], z);
 #pragma unroll
         for (int s = 0; s < SX; ++s) hn[h][s] = load(mh, lane + 64 * s, jh, z);
       }
