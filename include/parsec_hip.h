@@ -93,7 +93,9 @@ int pc_dense_destroy(void* ctx);
 int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, double* T, int ldt, double* lam_dev,
                 double* lam_host);
 int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
-/* eigensolver used by pc_dense_rr: 0 = divide and conquer (default), 1 = Jacobi */
+/* eigensolver used by pc_dense_rr: 0 = rocSOLVER divide and conquer (syevd),
+ * 1 = Jacobi (syevj), 2 = one-launch Householder tridiagonalization + rocSOLVER
+ * stedc + ormtr (default; k <= 512, larger k uses syevd) */
 int pc_dense_set_method(void* ctx, int method);
 
 /* ---- point-wise SCF kernels --------------------------------------------------

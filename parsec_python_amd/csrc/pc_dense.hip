@@ -10,6 +10,7 @@
 #include "pc_common.h"
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -23,7 +24,9 @@ struct DenseCtx {
   double* out;      // kmax + 4 : eigenvalues then 3 info codes (as doubles)
   int* info;        // 4
   double* host_out; // pinned, kmax + 4
-  int method;       // 0: divide and conquer (syevd), 1: Jacobi (syevj)
+  int method;       // 0: syevd, 1: Jacobi (syevj), 2: one-launch tridiagonalization + stedc + ormtr
+  double* tau;      // kmax (method 2: reflector scalars)
+  double* P;        // method 2: padded copy of G, 64*ceil(kmax/64) x (kmax+1)
   double* resid;    // syevj: off-diagonal norm at exit
   int* sweeps;      // syevj: sweeps used
 };
@@ -46,6 +49,203 @@ __global__ void zero_info(int* info) {
 
 __global__ void pack_out(const int* info, double* out, int k) {
   if (threadIdx.x < 3) out[k + threadIdx.x] = (double)info[threadIdx.x];
+}
+
+// ---- one-workgroup Householder tridiagonalization (LAPACK dsytrd layout, lower)
+//
+// For i = 0..n-2 the reflector H(i) = I - tau_i v_i v_i^T (v_i(i+1) = 1,
+// v_i(i+2:) stored in P(i+2:, i)) annihilates P(i+2:, i) of the trailing
+// matrix; d = diagonal, e = subdiagonal of the tridiagonal Q^T G Q with
+// Q = H(0)..H(n-2) -- the output of LAPACK's dsytd2 with uplo = 'L', so
+// rocSOLVER's ormtr applies Q.  The whole reduction is one launch (rocSOLVER's
+// sytrd spends most of its time launching per-column kernels at this size):
+//   pass i (all waves): over the trailing columns c >= i+1 (interleaved over
+//     the waves, U columns loaded per batch) apply the pending rank-2 update
+//     of step i-1 and accumulate p = P v_i; wave 0 keeps column i+1;
+//   serial part (wave 0 alone): w_i = tau_i p - (tau_i/2)(tau_i p.v_i) v_i,
+//     column i+1 brought up to date with (v_i, w_i), and the reflector of
+//     step i+1 built from it with wave-level sums.
+// P is a padded copy of G (ld = 64 * NCH rows, one extra all-zero column), so
+// the pass has no per-element conditions: a lane owns rows lane + 64 q of
+// every column, rows outside [i+1, n) are padding or stale upper-triangle
+// entries that nothing reads, and columns past n map to the zero column.
+// (Per-element conditions turn into branches whose joins make the compiler
+// wait for every outstanding store.)  P stays L2-resident for one workgroup.
+constexpr int kTrdMax = 512;                    // largest n
+constexpr int kTrdThreads = 512;
+constexpr int kTrdWaves = kTrdThreads / 64;
+
+__device__ __forceinline__ double wave_allsum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;   // the same in every lane (butterfly)
+}
+
+template <int NCH>
+__global__ __launch_bounds__(kTrdThreads) void sytrd_lower_1wg(const double* __restrict__ G, int ldg, int n,
+                                                               double* __restrict__ P, double* __restrict__ d,
+                                                               double* __restrict__ e, double* __restrict__ tau) {
+  constexpr int LDP = 64 * NCH;      // padded leading dimension of P
+  __shared__ double v[kTrdMax + 1], w[kTrdMax + 1], vn[kTrdMax + 1], colsh[kTrdMax];
+  __shared__ double red[kTrdWaves][kTrdMax];
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  for (int r = tid; r <= kTrdMax; r += kTrdThreads) {
+    v[r] = 0.0;
+    w[r] = 0.0;
+    vn[r] = 0.0;
+  }
+  // P <- G, zero padding rows and the extra column n
+  for (int c = wv; c <= n; c += kTrdWaves)
+    for (int r = lane; r < LDP; r += 64)
+      P[(long long)c * LDP + r] = (c < n && r < n) ? G[(long long)c * ldg + r] : 0.0;
+  __syncthreads();
+  double tau_cur = 0.0;   // wave 0: tau of the step being passed
+  double col[NCH];        // wave 0: up-to-date column j, rows lane + 64 q
+  // Reflector of column j from col[] (wave 0): d_j, e_j, tau_j, vn and P(:, j).
+  auto reflector = [&](int j) {
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) colsh[lane + 64 * q] = col[q];
+    __builtin_amdgcn_wave_barrier();
+    double* __restrict__ pj = P + (long long)j * LDP;
+    const double dj = colsh[j];
+    if (j == n - 1) {
+      if (lane == 0) {
+        pj[j] = dj;
+        d[j] = dj;
+      }
+      return;
+    }
+    const double alpha = colsh[j + 1];
+    double sq = 0.0;
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      const int r = lane + 64 * q;
+      sq += (r >= j + 2 && r < n) ? col[q] * col[q] : 0.0;
+    }
+    const double xnorm2 = wave_allsum(sq);
+    double beta = alpha, t = 0.0, scal = 0.0;
+    if (xnorm2 > 0.0) {
+      const double nrm = sqrt(alpha * alpha + xnorm2);
+      beta = alpha >= 0.0 ? -nrm : nrm;
+      t = (beta - alpha) / beta;
+      scal = 1.0 / (alpha - beta);
+    }
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      const int r = lane + 64 * q;
+      if (r >= j && r < n) {
+        const double out = r == j ? dj : (r == j + 1 ? beta : col[q] * scal);
+        pj[r] = out;
+        vn[r] = r == j ? 0.0 : (r == j + 1 ? 1.0 : out);
+      }
+    }
+    if (lane == 0) {
+      d[j] = dj;
+      e[j] = beta;
+      tau[j] = t;
+    }
+    tau_cur = t;
+  };
+  if (wv == 0) {
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) col[q] = P[lane + 64 * q];
+    reflector(0);
+  }
+  __syncthreads();
+  for (int i = 0; i + 1 < n; ++i) {
+    const double pf = i > 0 ? 1.0 : 0.0;   // a rank-2 update of step i-1 is pending
+    const int r0 = i + 1;
+    const int qb = r0 >> 6;                // first 64-row chunk holding rows >= i+1
+    // ---- pass i over the NQ chunks qb.. (one instantiation per NQ, no
+    // per-element conditions inside)
+    auto pass = [&](auto nq_c) {
+      constexpr int NQ = decltype(nq_c)::value;
+      constexpr int U = NQ == 1 ? 24 : 48 / NQ;   // columns per load batch
+      const int rb = 64 * qb + lane;
+      double pacc[NQ], vr[NQ], wr[NQ];
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        pacc[q] = 0.0;
+        vr[q] = pf * v[rb + 64 * q];
+        wr[q] = pf * w[rb + 64 * q];
+      }
+      for (int c0 = r0 + wv; c0 < n; c0 += kTrdWaves * U) {
+        double x[U][NQ];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int c = min(c0 + kTrdWaves * u, n);
+          const double* __restrict__ pc = P + (long long)c * LDP + rb;
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) x[u][q] = pc[64 * q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int c = min(c0 + kTrdWaves * u, n);
+          double* __restrict__ pc = P + (long long)c * LDP + rb;
+          const double vc = vn[c], pv = pf * v[c], pw = pf * w[c];
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) {
+            const double y = x[u][q] - (vr[q] * pw + wr[q] * pv);
+            pc[64 * q] = y;
+            pacc[q] += y * vc;
+            x[u][q] = y;
+          }
+        }
+        if (wv == 0 && c0 == r0) {   // wave 0's first column is i+1: keep it
+#pragma unroll
+          for (int q = 0; q < NQ; ++q) colsh[rb + 64 * q] = x[0][q];
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) red[wv][rb + 64 * q] = pacc[q];
+    };
+    switch (NCH - qb) {
+#define PC_TRD_PASS(N)                                    \
+  case N:                                                 \
+    if constexpr (N <= NCH) pass(std::integral_constant<int, N>{}); \
+    break;
+      PC_TRD_PASS(1) PC_TRD_PASS(2) PC_TRD_PASS(3) PC_TRD_PASS(4)
+      PC_TRD_PASS(5) PC_TRD_PASS(6) PC_TRD_PASS(7) PC_TRD_PASS(8)
+#undef PC_TRD_PASS
+      default: break;
+    }
+    __syncthreads();
+    // ---- serial part (wave 0): w_i, column i+1, reflector i+1
+    if (wv == 0) {
+      double wq[NCH], vq[NCH], dp = 0.0;
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        const int rr = lane + 64 * q;
+        const bool act = rr >= r0 && rr < n;
+        double p = 0.0;
+        if (act) {
+#pragma unroll
+          for (int ww = 0; ww < kTrdWaves; ++ww) p += red[ww][rr];
+        }
+        vq[q] = act ? vn[rr] : 0.0;
+        wq[q] = act ? tau_cur * p : 0.0;
+        col[q] = act ? colsh[rr] : 0.0;
+        dp += wq[q] * vq[q];
+      }
+      const double alpha2 = -0.5 * tau_cur * wave_allsum(dp);
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        const int rr = lane + 64 * q;
+        wq[q] += alpha2 * vq[q];
+        if (rr >= r0 && rr < n) {
+          w[rr] = wq[q];
+          v[rr] = vq[q];
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      // column i+1 (rows >= i+1): v_i(i+1) = 1
+      const double w1 = w[r0];
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) col[q] -= vq[q] * w1 + wq[q];
+      reflector(r0);
+    }
+    __syncthreads();
+  }
+  if (n == 1 && tid == 0) d[0] = P[0];
 }
 
 #define PC_RB(call)                                                                          \
@@ -79,8 +279,12 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipMalloc(&c->info, 4 * sizeof(int)));
   PC_HIP(hipMalloc(&c->resid, sizeof(double)));
   PC_HIP(hipMalloc(&c->sweeps, sizeof(int)));
+  PC_HIP(hipMalloc(&c->tau, (size_t)kmax * sizeof(double)));
+  c->P = nullptr;
+  if (kmax <= kTrdMax)
+    PC_HIP(hipMalloc(&c->P, (size_t)(64 * ((kmax + 63) / 64)) * (kmax + 1) * sizeof(double)));
   PC_HIP(hipHostMalloc(&c->host_out, (size_t)(kmax + 4) * sizeof(double), hipHostMallocDefault));
-  c->method = 0;
+  c->method = 2;
   *out_ctx = c;
   return pc::PC_OK;
 }
@@ -90,7 +294,7 @@ int pc_dense_destroy(void* ctx) {
   if (!c) return pc::PC_OK;
   rocblas_destroy_handle(c->h);
   (void)hipFree(c->tmp); (void)hipFree(c->E); (void)hipFree(c->out); (void)hipFree(c->info);
-  (void)hipFree(c->resid); (void)hipFree(c->sweeps);
+  (void)hipFree(c->resid); (void)hipFree(c->sweeps); (void)hipFree(c->tau); (void)hipFree(c->P);
   (void)hipHostFree(c->host_out);
   delete c;
   return pc::PC_OK;
@@ -118,15 +322,36 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
                       rocblas_diagonal_non_unit, k, k, &one, S, lds, c->tmp, k, Gy, ldg));
   hipLaunchKernelGGL(symmetrize, dim3((k * k + 255) / 256), dim3(256), 0, c->stream, Gy, k, ldg);
   PC_LAUNCH_CHECK();
-  if (c->method == 1)
+  const double* Q = Gy;   // eigenvectors of G
+  int ldq = ldg;
+  if (c->method == 2 && c->P && k <= kTrdMax) {
+    // G = H T H^T (one launch), T = Z diag(lam) Z^T (stedc), Q = H Z (ormtr)
+    const int nch = (k + 63) / 64;
+    switch (nch) {
+#define PC_TRD(N)                                                                                                  \
+  case N:                                                                                                          \
+    hipLaunchKernelGGL(sytrd_lower_1wg<N>, dim3(1), dim3(kTrdThreads), 0, c->stream, Gy, ldg, k, c->P, c->out,  \
+                       c->E, c->tau);                                                                              \
+    break;
+      PC_TRD(1) PC_TRD(2) PC_TRD(3) PC_TRD(4) PC_TRD(5) PC_TRD(6) PC_TRD(7) PC_TRD(8)
+#undef PC_TRD
+    }
+    PC_LAUNCH_CHECK();
+    PC_RB(rocsolver_dstedc(c->h, rocblas_evect_tridiagonal, k, c->out, c->E, c->tmp, k, c->info + 2));
+    PC_RB(rocsolver_dormtr(c->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, k, k, c->P, 64 * nch,
+                           c->tau, c->tmp, k));
+    Q = c->tmp;
+    ldq = k;
+  } else if (c->method == 1) {
     PC_RB(rocsolver_dsyevj(c->h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg,
                            0.0, c->resid, 100, c->sweeps, c->out, c->info + 2));
-  else
+  } else {
     PC_RB(rocsolver_dsyevd(c->h, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg, c->out, c->E,
                            c->info + 2));
+  }
   // T = R^-1 Q
   PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                      rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg, T, ldt));
+                      rocblas_diagonal_non_unit, k, k, &one, S, lds, Q, ldq, T, ldt));
   PC_HIP(hipMemcpyAsync(lam_dev, c->out, k * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
   hipLaunchKernelGGL(pack_out, dim3(1), dim3(64), 0, c->stream, c->info, c->out, k);
   PC_LAUNCH_CHECK();
@@ -148,11 +373,12 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
 
 // Eigensolver of the projected k x k problem: 0 = divide and conquer
 // (rocSOLVER syevd), 1 = cyclic Jacobi (rocSOLVER syevj, tolerance = machine
-// precision, at most 100 sweeps).
+// precision, at most 100 sweeps), 2 = one-launch Householder tridiagonalization
+// (k <= 512; larger k uses syevd) + rocSOLVER stedc + ormtr.
 int pc_dense_set_method(void* ctx, int method) {
   DenseCtx* c = static_cast<DenseCtx*>(ctx);
   PC_CHECK_ARG(c, "null context");
-  PC_CHECK_ARG(method == 0 || method == 1, "method must be 0 (syevd) or 1 (syevj)");
+  PC_CHECK_ARG(method >= 0 && method <= 2, "method must be 0 (syevd), 1 (syevj) or 2 (tridiagonalization + stedc)");
   c->method = method;
   return pc::PC_OK;
 }

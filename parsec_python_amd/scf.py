@@ -65,10 +65,11 @@ class SCFConfig:
     polish: bool = True
     polish_de: float = 5e-6
     polish_err: float | None = None
-    # eigensolver of the k x k Rayleigh-Ritz problem: "syevd" (divide and
-    # conquer) or "syevj" (Jacobi; fast once the projected matrix is nearly
-    # diagonal, as it is after the first few SCF iterations)
-    dense_method: str = "syevd"
+    # eigensolver of the k x k Rayleigh-Ritz problem: "tridiag" (one-launch
+    # Householder tridiagonalization + divide and conquer, k <= 512; larger k
+    # falls back to "syevd"), "syevd" (rocSOLVER divide and conquer) or
+    # "syevj" (Jacobi).  All three return the same eigenpairs to rounding.
+    dense_method: str = "tridiag"
     # extra fixed-potential filter passes after first_filt when the SCF starts
     # from a given (learned) density, until the mean occupied Ritz value moves
     # by less than start_warmup_tol Ha (see run_scf).  Off by default: on
@@ -176,7 +177,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
 
     H.dtype = prob.dtype
     eig = Eigensolver(H, k, seed=cfg.seed)
-    dense_method = {"syevd": 0, "syevj": 1}[cfg.dense_method]
+    dense_method = {"syevd": 0, "syevj": 1, "tridiag": 2}[cfg.dense_method]
     _lib.call("pc_dense_set_method", eig.la.dense, dense_method)
     hart = HartreeSolver(H.dgrid, h, reference_aliasing=cfg.pcg_aliasing, prec=cfg.pcg_prec)
     mixer = DeviceMsecant1(n, cfg.maxits + 2)

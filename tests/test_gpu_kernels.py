@@ -198,6 +198,47 @@ def test_dense_rr_generalized_eig(gpu):
     la.close()
 
 
+@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("k,kind", [(1, "rand"), (2, "rand"), (3, "rand"), (65, "rand"), (275, "degenerate"),
+                                    (300, "rand"), (520, "rand")])
+def test_dense_rr_methods(gpu, method, k, kind):
+    """Every eigensolver of the dense stage (syevd, syevj, one-launch
+    tridiagonalization + stedc) returns the generalized eigenpairs of (G, S):
+    T^T S T = I, T^T G T = diag(lam), lam = SciPy's.  'degenerate' has exact
+    triple eigenvalues, as the symmetric clusters give."""
+    import torch
+    from scipy.linalg import eigh
+    from parsec_python_amd import _lib
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    if method == 1 and k > 300:
+        pytest.skip("Jacobi at this size is slow and covered at k <= 300")
+    print(f"dense_rr method {method} k {k} {kind}", flush=True)
+    rng = np.random.default_rng(k + 7 * method)
+    B = rng.standard_normal((max(2 * k, 4), k))
+    S = B.T @ B / B.shape[0] + 0.5 * np.eye(k)
+    if kind == "degenerate":
+        Qr, _ = np.linalg.qr(rng.standard_normal((k, k)))
+        lam0 = np.repeat(rng.uniform(-1.0, 2.0, (k + 2) // 3), 3)[:k]
+        A = (Qr * lam0) @ Qr.T
+        L = np.linalg.cholesky(S)
+        G = L @ A @ L.T                      # (G, S) has the spectrum lam0
+    else:
+        M = rng.standard_normal((k, k))
+        G = M + M.T
+    la = BlockLinalg(10, k)
+    _lib.call("pc_dense_set_method", la.dense, method)
+    Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
+    Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
+    T, lam = la.dense_rr(Sd, Gd, k)
+    ref = eigh(G, S, eigvals_only=True)
+    scale = max(1.0, np.abs(ref).max())
+    assert np.allclose(lam, ref, rtol=0, atol=1e-11 * scale * k)
+    Tm = T.cpu().numpy()[:k * k].reshape(k, k).T  # column-major
+    assert np.abs(Tm.T @ S @ Tm - np.eye(k)).max() < 1e-10 * k
+    assert np.abs(Tm.T @ G @ Tm - np.diag(lam)).max() < 1e-10 * scale * k
+    la.close()
+
+
 def test_density_xc_post_vs_oracle(gpu):
     import torch
     from parsec_python_amd import _lib
