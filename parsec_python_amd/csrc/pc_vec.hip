@@ -264,9 +264,11 @@ int pc_reduce_sets(const double* part, int nblk, int nsets, double* out, void* s
 // Workspace for pc_dots: m * pc_dots_nchunk(n) doubles.
 int pc_dots_nchunk(long long n, int* out) {
   PC_CHECK_ARG(out && n >= 1, "bad argument");
-  long long nch = (n + 65535) / 65536;
+  // one workgroup per 2048 points (8 per thread): a single dot over 605,808
+  // points takes 296 workgroups (10 at 65,536 points each took 70 us)
+  long long nch = (n + 2047) / 2048;
   if (nch < 1) nch = 1;
-  if (nch > 256) nch = 256;
+  if (nch > 1024) nch = 1024;
   *out = (int)nch;
   return pc::PC_OK;
 }
