@@ -159,10 +159,9 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
     const int l = z - w.wb;
     return make_int2(__builtin_amdgcn_readlane(w.v.x, l), __builtin_amdgcn_readlane(w.v.y, l));
   };
-  static_assert(TYW == 1, "metadata windows hold one own row per wave");
-  Win wown, whalo[HT];
-  const int jown = j0 + wv;
-  win_load(wown, jown, zb - R);
+  Win wown[TYW], whalo[HT];
+#pragma unroll
+  for (int r = 0; r < TYW; ++r) win_load(wown[r], j0 + wv + NW * r, zb - R);
 #pragma unroll
   for (int t = 0; t < HT; ++t) {
     const int hr = wv + NW * t;
@@ -191,7 +190,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
       for (int s = 0; s < SX; ++s) q[u][r][s][0] = T(0);
 #pragma unroll
     for (int dz = 1; dz <= 2 * R; ++dz) {
-      const int2 m = win_at(wown, zb - R - 1 + dz);
+      const int2 m = win_at(wown[r], zb - R - 1 + dz);
 #pragma unroll
       for (int u = 0; u < NV; ++u)
 #pragma unroll
@@ -201,11 +200,11 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   // issue everything plane z needs into stage S
   auto prefetch = [&](auto stage, int z) {
     constexpr int S = decltype(stage)::value;
-    if (z + R >= wown.wb + 64) win_load(wown, jown, z);
 #pragma unroll
     for (int r = 0; r < TYW; ++r) {
-      const int2 mq = win_at(wown, z + R);
-      const int2 mo = win_at(wown, z);
+      if (z + R >= wown[r].wb + 64) win_load(wown[r], j0 + wv + NW * r, z);
+      const int2 mq = win_at(wown[r], z + R);
+      const int2 mo = win_at(wown[r], z);
       mnext[S][r] = mo;
       const int lo = mo.y & 0xffff, hi = mo.y >> 16;
 #pragma unroll

@@ -196,7 +196,10 @@ class Eigensolver:
         return W, lam
 
     # -------------------------------------------------------------- drivers
-    def first_filt(self, k: int, polm: int = 10):
+    def first_filt(self, k: int, polm: int = 10, start_weight: torch.Tensor | None = None):
+        """first_filt.py: random start block, slim Chebyshev filter + Rayleigh-Ritz
+        passes until the Ritz-value trace settles.  ``start_weight`` (per point,
+        optional) multiplies the random start vectors point-wise."""
         energ_tol = 0.08
         if polm < 10 or polm > 15:
             polm = 10
@@ -205,6 +208,8 @@ class Eigensolver:
         lowerb = 0.8 * ritzv[0] + 0.2 * ritzv[-2]
         W = self.bufs[0]
         rand_block(W, self.n, k, self.seeds.next(), normal=False)
+        if start_weight is not None:
+            W[:k, :self.n] *= start_weight.to(W.dtype)
         tr0 = None
         outer = 0
         for it in range(1, max_iter + 1):

@@ -76,6 +76,12 @@ class SCFConfig:
     # Si87H76 3 passes save 2 SCF iterations and cost about as much.
     start_warmup: int = 0
     start_warmup_tol: float = 1e-3
+    # with a given start density (learned start): first_filt's random start
+    # vectors are weighted point-wise by sqrt(rho_init / max rho_init), so the
+    # start block already leans to where the electrons are.  On Si87H76 (seeds
+    # 1-4) this takes the mixed solve from 17 to 15 SCF iterations; the energy
+    # at the stop stays within the spread the unweighted start shows.
+    weighted_start: bool = True
     verbose: bool = False
 
 
@@ -199,7 +205,11 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         t0 = time.perf_counter()
         H.set_potential(pot)
         if its == 1:
-            W, lam = eig.first_filt(k, cfg.poldeg)
+            sw = None
+            if rho_init is not None and cfg.weighted_start:
+                r = torch.clamp(rho_init, min=0.0)
+                sw = torch.sqrt(r / r.max())
+            W, lam = eig.first_filt(k, cfg.poldeg, start_weight=sw)
             first_filt_info = dict(eig.last_first_filt)
             nwarm = (cfg.start_warmup if rho_init is not None else 0) if warmup is None else warmup
             mean_prev = float(np.mean(lam[:nev]))
