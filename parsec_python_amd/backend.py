@@ -92,7 +92,8 @@ def _radius_of(A) -> int:
 class GpuContext:
     """Device state shared by the backend stages of one run (one grid, one atom set)."""
 
-    def __init__(self, domain: dict, atoms, precision: str = "fp64", seed: int = 0, pcg_aliasing: bool = True):
+    def __init__(self, domain: dict, atoms, precision: str = "fp64", seed: int = 0, pcg_aliasing: bool = True,
+                 pcg_prec: str = "none"):
         require_gpu()
         nx, ny, nz = int(domain["nx"]), int(domain["ny"]), int(domain["nz"])
         if not (nx == ny == nz):
@@ -105,6 +106,7 @@ class GpuContext:
         self.precision = precision
         self.seed = seed
         self.pcg_aliasing = pcg_aliasing
+        self.pcg_prec = pcg_prec
         self.radius = None
         self.H = None
         self.eig = None
@@ -134,7 +136,8 @@ class GpuContext:
     def hartree(self, radius: int) -> HartreeSolver:
         if radius not in self._pcg_grid:
             dg = self.H.dgrid if (self.H is not None and self.radius == radius) else DeviceGrid(self.grid, radius)
-            self._pcg_grid[radius] = HartreeSolver(dg, self.h, reference_aliasing=self.pcg_aliasing)
+            self._pcg_grid[radius] = HartreeSolver(dg, self.h, reference_aliasing=self.pcg_aliasing,
+                                                   prec=self.pcg_prec)
         return self._pcg_grid[radius]
 
     def check_laplacian(self, A, scale: float) -> int:
@@ -230,9 +233,12 @@ class KohnShamOperator:
 class GpuBackendImpl:
     """The stage functions, bound to one GpuContext."""
 
-    def __init__(self, precision: str = "fp64", seed: int = 0, pcg_aliasing: bool = True):
+    def __init__(self, precision: str = "fp64", seed: int = 0, pcg_aliasing: bool = True, pcg_prec: str = "none"):
         if precision not in ("fp64", "mixed"):
             raise ValueError("precision must be 'fp64' or 'mixed'")
+        if pcg_prec not in ("none", "mg"):
+            raise ValueError("pcg_prec must be 'none' or 'mg'")
+        self.pcg_prec = pcg_prec
         self.precision = precision
         self.seed = seed
         self.pcg_aliasing = pcg_aliasing
@@ -241,7 +247,7 @@ class GpuBackendImpl:
 
     def _context(self, domain, atoms) -> GpuContext:
         if self.ctx is None:
-            self.ctx = GpuContext(domain, atoms, self.precision, self.seed, self.pcg_aliasing)
+            self.ctx = GpuContext(domain, atoms, self.precision, self.seed, self.pcg_aliasing, self.pcg_prec)
         return self.ctx
 
     # -- src/V_ion/pseudoDiag.py:44-421
@@ -331,8 +337,9 @@ class GpuBackendImpl:
         raise NotImplementedError("chsubsp (diagmeth 2) is not provided by the HIP backend; use diagmeth 3")
 
 
-def make_backend(precision: str = "fp64", seed: int = 0, pcg_aliasing: bool = True) -> SolverBackend:
-    impl = GpuBackendImpl(precision, seed, pcg_aliasing)
+def make_backend(precision: str = "fp64", seed: int = 0, pcg_aliasing: bool = True,
+                 pcg_prec: str = "none") -> SolverBackend:
+    impl = GpuBackendImpl(precision, seed, pcg_aliasing, pcg_prec)
     return SolverBackend(label="gpu", pseudo_diag=impl.pseudo_diag, pseudo_nl=impl.pseudo_nl,
                          first_filt=impl.first_filt, chefsi1=impl.chefsi1, lanczos=impl.lanczos,
                          chsubsp=impl.chsubsp, occupations=impl.occupations, pcg=impl.pcg, array_module=torch,
@@ -344,7 +351,9 @@ def select_solver_backend(settings) -> SolverBackend:
 
     Optional settings attributes: ``gpu_precision`` ("fp64" | "mixed"),
     ``gpu_seed`` (start-vector seed), ``gpu_pcg_aliasing`` (reproduce the CPU
-    CG's first-iteration aliasing; default True).
+    CG's first-iteration aliasing; default True), ``gpu_pcg_prec`` ("none",
+    the reference's default unpreconditioned CG, or "mg": multigrid-
+    preconditioned CG with the same stop rule).
     """
     if not getattr(settings, "use_gpu", 0):
         raise ValueError("this backend is the GPU path (use_gpu=1); the CPU path is the reference's own bundle")
@@ -355,4 +364,4 @@ def select_solver_backend(settings) -> SolverBackend:
         raise SystemExit(f"GPU backend requested but the HIP library could not be loaded: {e}")
     require_gpu()
     return make_backend(getattr(settings, "gpu_precision", "fp64"), getattr(settings, "gpu_seed", 0),
-                        getattr(settings, "gpu_pcg_aliasing", True))
+                        getattr(settings, "gpu_pcg_aliasing", True), getattr(settings, "gpu_pcg_prec", "none"))

@@ -37,3 +37,14 @@ def test_select_backend_without_gpu_fails_loudly():
     from parsec_python_amd.backend import select_solver_backend
     with pytest.raises((SystemExit, RuntimeError)):
         select_solver_backend(types.SimpleNamespace(use_gpu=1))
+
+
+def test_backend_options_validated():
+    """Unknown precision or Hartree preconditioner names are rejected before any GPU work."""
+    from parsec_python_amd.backend import GpuBackendImpl
+    with pytest.raises(ValueError):
+        GpuBackendImpl("fp16")
+    with pytest.raises(ValueError):
+        GpuBackendImpl("mixed", pcg_prec="ilu")
+    impl = GpuBackendImpl("mixed", pcg_prec="mg")
+    assert impl.pcg_prec == "mg" and impl.ctx is None
