@@ -156,6 +156,134 @@ __global__ __launch_bounds__(256) void gram_kernel(const T* __restrict__ X, long
       }
 }
 
+// FP32 form with the operand tiles staged in LDS: per stage of 32 points the
+// workgroup loads its 64 X columns and 64 Y columns once (two float4 per thread
+// per operand) into one of two LDS buffers, and each wave reads its 32-column
+// halves from there, so every operand value comes from L2 once per workgroup
+// instead of once per wave.  The next stage's global loads are in flight while
+// the MFMAs run; one barrier per stage.  Same MFMA inputs and the same FP64
+// flush every 16 points as gram_kernel<float, true>, so the results match it.
+// Requires 16-byte aligned columns (ld % 4 == 0) and chunk % 32 == 0.
+__global__ __launch_bounds__(256) void gram_lds_f32(const float* __restrict__ X, long long ldx,
+                                                    const float* __restrict__ Y, long long ldy, long long n, int kx,
+                                                    int ky, int ti_n, int tj_n, int ntile, int sym, int nchunk,
+                                                    long long chunk, double* __restrict__ part) {
+  constexpr int SP = 32;        // points per stage
+  constexpr int LS = SP + 4;    // LDS row stride (floats)
+  __shared__ float xs[2][64 * LS], ys[2][64 * LS];
+  const int b0 = blockIdx.x;
+  const int tile = b0 % ntile;
+  const int ch = b0 / ntile;
+  int ti, tj;
+  tile_coords(tile, ti_n, tj_n, sym, ti, tj);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wi = w >> 1, wj = w & 1;
+  const int lm = lane & 15, lk = lane >> 4;
+  const int ib = ti * 64, jb = tj * 64;
+  // loader: thread -> column lc of the tile, 8-point segment ls of the stage
+  const int lc = threadIdx.x >> 2, ls = (threadIdx.x & 3) * 8;
+  const bool xin = ib + lc < kx, yin = jb + lc < ky;
+  const float* __restrict__ xcol = X + (long long)(xin ? ib + lc : 0) * ldx;
+  const float* __restrict__ ycol = Y + (long long)(yin ? jb + lc : 0) * ldy;
+  const long long pbeg = (long long)ch * chunk;
+  const long long pend = min(n, pbeg + chunk);
+  float gx[8], gy[8];
+  auto gload = [&](long long p) {
+    const long long q = p + ls;
+    if (q + 8 <= pend) {
+      if (xin) {
+        const float4 t0 = *reinterpret_cast<const float4*>(xcol + q);
+        const float4 t1 = *reinterpret_cast<const float4*>(xcol + q + 4);
+        gx[0] = t0.x; gx[1] = t0.y; gx[2] = t0.z; gx[3] = t0.w; gx[4] = t1.x; gx[5] = t1.y; gx[6] = t1.z; gx[7] = t1.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) gx[e] = 0.0f;
+      }
+      if (yin) {
+        const float4 t0 = *reinterpret_cast<const float4*>(ycol + q);
+        const float4 t1 = *reinterpret_cast<const float4*>(ycol + q + 4);
+        gy[0] = t0.x; gy[1] = t0.y; gy[2] = t0.z; gy[3] = t0.w; gy[4] = t1.x; gy[5] = t1.y; gy[6] = t1.z; gy[7] = t1.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) gy[e] = 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        gx[e] = (xin && q + e < pend) ? xcol[q + e] : 0.0f;
+        gy[e] = (yin && q + e < pend) ? ycol[q + e] : 0.0f;
+      }
+    }
+  };
+  auto sstore = [&](int bf) {
+    float* xd = &xs[bf][lc * LS + ls];
+    float* yd = &ys[bf][lc * LS + ls];
+    *reinterpret_cast<float4*>(xd) = make_float4(gx[0], gx[1], gx[2], gx[3]);
+    *reinterpret_cast<float4*>(xd + 4) = make_float4(gx[4], gx[5], gx[6], gx[7]);
+    *reinterpret_cast<float4*>(yd) = make_float4(gy[0], gy[1], gy[2], gy[3]);
+    *reinterpret_cast<float4*>(yd + 4) = make_float4(gy[4], gy[5], gy[6], gy[7]);
+  };
+  v4f acc[2][2];
+  double acc64[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      acc[a][c] = v4f{0, 0, 0, 0};
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc64[a][c][r] = 0.0;
+    }
+  if (pbeg < pend) {
+    gload(pbeg);
+    sstore(0);
+  }
+  __syncthreads();
+  int bf = 0;
+  for (long long p = pbeg; p < pend; p += SP) {
+    const bool more = p + SP < pend;
+    if (more) gload(p + SP);
+#pragma unroll
+    for (int g = 0; g < SP / 16; ++g) {
+      float a[2][4], bb[2][4];
+#pragma unroll
+      for (int t = 0; t < 2; ++t) {
+        const float4 av = *reinterpret_cast<const float4*>(&xs[bf][(wi * 32 + 16 * t + lm) * LS + 16 * g + 4 * lk]);
+        const float4 bv = *reinterpret_cast<const float4*>(&ys[bf][(wj * 32 + 16 * t + lm) * LS + 16 * g + 4 * lk]);
+        a[t][0] = av.x; a[t][1] = av.y; a[t][2] = av.z; a[t][3] = av.w;
+        bb[t][0] = bv.x; bb[t][1] = bv.y; bb[t][2] = bv.z; bb[t][3] = bv.w;
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < 4; ++s2)
+#pragma unroll
+        for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+          for (int tb = 0; tb < 2; ++tb) acc[ta][tb] = Acc<float>::mfma(a[ta][s2], bb[tb][s2], acc[ta][tb]);
+#pragma unroll
+      for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+        for (int tb = 0; tb < 2; ++tb) {
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc64[ta][tb][r] += (double)acc[ta][tb][r];
+          acc[ta][tb] = v4f{0, 0, 0, 0};
+        }
+    }
+    if (more) sstore(bf ^ 1);
+    __syncthreads();
+    bf ^= 1;
+  }
+  double* out = part + ((long long)ch * ntile + tile) * 4096;
+#pragma unroll
+  for (int ta = 0; ta < 2; ++ta)
+#pragma unroll
+    for (int tb = 0; tb < 2; ++tb)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int ii = wi * 32 + 16 * ta + Acc<float>::row(lk, r);
+        const int jj = wj * 32 + 16 * tb + lm;
+        out[jj * 64 + ii] = acc64[ta][tb][r];
+      }
+}
+
 // 16 workgroups per 64x64 tile, one element per thread; chunks summed in
 // order (deterministic)
 __global__ void gram_reduce(const double* __restrict__ part, int ntile, int nchunk, int ti_n, int tj_n, int kx,
@@ -254,7 +382,7 @@ GramPlan plan_gram(long long n, int kx, int ky, int sym) {
   long long maxch = std::max<long long>(1, (n + 1023) / 1024);
   p.nchunk = (int)std::min(target, maxch);
   long long chunk = (n + p.nchunk - 1) / p.nchunk;
-  chunk = (chunk + 15) / 16 * 16;
+  chunk = (chunk + 31) / 32 * 32;   // whole 32-point stages of gram_lds_f32
   p.chunk = chunk;
   p.nchunk = (int)((n + chunk - 1) / chunk);
   if (p.nchunk < 1) p.nchunk = 1;
@@ -278,6 +406,17 @@ int gram_impl(const T* X, long long ldx, const T* Y, long long ldy, long long n,
                    (reinterpret_cast<uintptr_t>(X) % (4 * sizeof(T)) == 0) &&
                    (reinterpret_cast<uintptr_t>(Y) % (4 * sizeof(T)) == 0);
   dim3 grid(p.ntile * p.nchunk);
+  if constexpr (sizeof(T) == 4) {
+    if (vec) {
+      hipLaunchKernelGGL(gram_lds_f32, grid, dim3(256), 0, st, X, ldx, Y, ldy, n, kx, ky, p.ti_n, p.tj_n, p.ntile,
+                         sym, p.nchunk, p.chunk, part);
+      PC_LAUNCH_CHECK();
+      hipLaunchKernelGGL(gram_reduce, dim3(16 * p.ntile), dim3(256), 0, st, part, p.ntile, p.nchunk, p.ti_n, p.tj_n,
+                         kx, ky, S, lds, sym);
+      PC_LAUNCH_CHECK();
+      return pc::PC_OK;
+    }
+  }
   if (vec)
     hipLaunchKernelGGL((gram_kernel<T, true>), grid, dim3(256), 0, st, X, ldx, Y, ldy, n, kx, ky, p.ti_n, p.tj_n,
                        p.ntile, sym, p.nchunk, p.chunk, part);
