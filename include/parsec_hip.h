@@ -125,6 +125,16 @@ int pc_dots(long long n, const double* A, long long lda, int m, const double* v,
             void* stream);
 int pc_lincomb(long long n, const double* A, long long lda, int m, const double* coef_host, double alpha,
                const double* x, double beta, double* y, void* stream);
+/* Lanczos bound without host round trips (replaces the vdot / norm / axpy
+ * sequence of src/Eigensolvers/lanczosForChefsi1.py:5-99): pc_lanczos_start
+ * normalises v; after w = H v1, pc_lanczos_step(k) computes alpha_k = v1.w,
+ * w -= alpha_k v1 + beta_{k-1} v0, beta_k = |w|, w /= beta_k and stores
+ * coef[2k] = alpha_k, coef[2k+1] = beta_k on the device.  part: 2 *
+ * pc_lanczos_nblocks(n) doubles. */
+int pc_lanczos_nblocks(long long n, int* out);
+int pc_lanczos_start(long long n, double* v, double* part, void* stream);
+int pc_lanczos_step(long long n, const double* v0, const double* v1, double* w, double* coef, int k, double* part,
+                    void* stream);
 int pc_convert_f64_to_f32(long long n, int ncol, const double* src, long long lds, float* dst, long long ldd,
                           void* stream);
 int pc_convert_f32_to_f64(long long n, int ncol, const float* src, long long lds, double* dst, long long ldd,

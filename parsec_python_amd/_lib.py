@@ -67,6 +67,9 @@ PROTOTYPES = {
     "pc_dots_nchunk": [_ll, _pi],
     "pc_dots": [_ll, _vp, _ll, _i, _vp, _vp, _vp, _vp],
     "pc_lincomb": [_ll, _vp, _ll, _i, _pd, _d, _vp, _d, _vp, _vp],
+    "pc_lanczos_nblocks": [_ll, _pi],
+    "pc_lanczos_start": [_ll, _vp, _vp, _vp],
+    "pc_lanczos_step": [_ll, _vp, _vp, _vp, _vp, _i, _vp, _vp],
     "pc_convert_f64_to_f32": [_ll, _i, _vp, _ll, _vp, _ll, _vp],
     "pc_convert_f32_to_f64": [_ll, _i, _vp, _ll, _vp, _ll, _vp],
     "pc_rand_f64": [_ll, _i, _ll, _u64, _i, _vp, _vp],

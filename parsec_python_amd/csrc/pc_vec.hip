@@ -191,6 +191,71 @@ __global__ __launch_bounds__(kBlk) void rand_kernel(long long n, int ncol, long 
 
 inline unsigned blocks(long long n) { return (unsigned)((n + kBlk - 1) / kBlk); }
 
+// ---------------------------------------------------------------- Lanczos
+// Device-side three-term Lanczos step (the bound of src/Eigensolvers/
+// lanczosForChefsi1.py:5-99) without host round trips: alpha and beta stay in
+// coef[] on the device and the host reads all of them once.  One workgroup per
+// kLzChunk points; every workgroup sums the partials of the previous stage
+// itself, in one fixed order, so all hold the same scalar.
+constexpr int kLzChunk = 2048;
+
+inline unsigned lz_blocks(long long n) { return (unsigned)((n + kLzChunk - 1) / kLzChunk); }
+
+__device__ __forceinline__ double lz_sum_parts(const double* __restrict__ part, int nb, double* red) {
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlk) s += part[b];
+  const double t = pc_block_sum(s, red);
+  __shared__ double bc;
+  if (threadIdx.x == 0) bc = t;
+  __syncthreads();
+  return bc;
+}
+
+// part[b] = sum over chunk b of a[p] * c[p]
+__global__ __launch_bounds__(kBlk) void lz_dot(long long n, const double* __restrict__ a, const double* __restrict__ c,
+                                               double* __restrict__ part) {
+  __shared__ double red[kBlk / 64];
+  const long long p0 = (long long)blockIdx.x * kLzChunk, p1 = min(n, p0 + kLzChunk);
+  double s = 0.0;
+  for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) s += a[p] * c[p];
+  const double t = pc_block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// alpha = sum(pa) (v1 . H v1); w -= alpha v1 + beta v0 (beta = coef[2k-1], 0 at
+// k = 0); pb = partials of w . w; coef[2k] = alpha
+__global__ __launch_bounds__(kBlk) void lz_orth(long long n, const double* __restrict__ v0,
+                                                const double* __restrict__ v1, double* __restrict__ w,
+                                                double* __restrict__ coef, int k, const double* __restrict__ pa,
+                                                double* __restrict__ pb, int nb) {
+  __shared__ double red[kBlk / 64];
+  const double alpha = lz_sum_parts(pa, nb, red);
+  const double beta = k > 0 ? coef[2 * k - 1] : 0.0;
+  const long long p0 = (long long)blockIdx.x * kLzChunk, p1 = min(n, p0 + kLzChunk);
+  double s = 0.0;
+  for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) {
+    const double x = w[p] - alpha * v1[p] - beta * v0[p];
+    w[p] = x;
+    s += x * x;
+  }
+  const double t = pc_block_sum(s, red);
+  if (threadIdx.x == 0) {
+    pb[blockIdx.x] = t;
+    if (blockIdx.x == 0) coef[2 * k] = alpha;
+  }
+}
+
+// nrm = sqrt(sum(pb)); v /= nrm; *slot = nrm (slot may be null)
+__global__ __launch_bounds__(kBlk) void lz_norm(long long n, double* __restrict__ v, const double* __restrict__ pb,
+                                                int nb, double* __restrict__ slot) {
+  __shared__ double red[kBlk / 64];
+  const double nrm = sqrt(lz_sum_parts(pb, nb, red));
+  const double inv = 1.0 / nrm;
+  const long long p0 = (long long)blockIdx.x * kLzChunk, p1 = min(n, p0 + kLzChunk);
+  for (long long p = p0 + threadIdx.x; p < p1; p += kBlk) v[p] *= inv;
+  if (slot && blockIdx.x == 0 && threadIdx.x == 0) *slot = nrm;
+}
+
 }  // namespace
 
 extern "C" {
@@ -315,6 +380,43 @@ int pc_convert_f32_to_f64(long long n, int ncol, const float* src, long long lds
   PC_CHECK_ARG(n >= 1 && ncol >= 1 && ncol <= 65535 && src && dst, "bad argument");
   hipLaunchKernelGGL((convert_kernel<float, double>), dim3(blocks(n), ncol), dim3(kBlk), 0, (hipStream_t)stream, n,
                      ncol, src, lds, dst, ldd);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+// Lanczos workspace: 2 * pc_lanczos_nblocks(n) doubles.
+int pc_lanczos_nblocks(long long n, int* out) {
+  PC_CHECK_ARG(out && n >= 1, "bad argument");
+  *out = (int)lz_blocks(n);
+  return pc::PC_OK;
+}
+
+// v <- v / ||v|| (device only).
+int pc_lanczos_start(long long n, double* v, double* part, void* stream) {
+  PC_CHECK_ARG(n >= 1 && v && part, "bad argument");
+  const unsigned nb = lz_blocks(n);
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(lz_dot, dim3(nb), dim3(kBlk), 0, st, n, v, v, part);
+  PC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(lz_norm, dim3(nb), dim3(kBlk), 0, st, n, v, part, (int)nb, (double*)nullptr);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+// Step k (0-based) after w = H v1: alpha_k = v1 . w, w -= alpha_k v1 + beta_{k-1} v0,
+// beta_k = ||w||, w /= beta_k; coef[2k] = alpha_k, coef[2k+1] = beta_k (device).
+int pc_lanczos_step(long long n, const double* v0, const double* v1, double* w, double* coef, int k, double* part,
+                    void* stream) {
+  PC_CHECK_ARG(n >= 1 && v0 && v1 && w && coef && part && k >= 0, "bad argument");
+  const unsigned nb = lz_blocks(n);
+  hipStream_t st = (hipStream_t)stream;
+  double* pa = part;
+  double* pb = part + nb;
+  hipLaunchKernelGGL(lz_dot, dim3(nb), dim3(kBlk), 0, st, n, v1, w, pa);
+  PC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(lz_orth, dim3(nb), dim3(kBlk), 0, st, n, v0, v1, w, coef, k, pa, pb, (int)nb);
+  PC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(lz_norm, dim3(nb), dim3(kBlk), 0, st, n, w, pb, (int)nb, coef + 2 * k + 1);
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
 }

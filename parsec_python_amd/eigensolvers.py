@@ -19,13 +19,14 @@ an explicit seed (the reference is unseeded).
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 
 import numpy as np
 import torch
 
 from . import _lib
-from .device import F64, ptr, stream
+from .device import F64, Scratch, ptr, stream
 from .hamiltonian import BlockLinalg, Hamiltonian, VecOps
 
 
@@ -57,6 +58,10 @@ class Eigensolver:
         self.la = BlockLinalg(self.n, kmax)
         self.bufs = [H.new_block(kmax) for _ in range(3)]
         self.lz = torch.empty((3, self.ld), dtype=F64, device="cuda")  # Lanczos vectors (FP64)
+        nbl = C.c_int()
+        _lib.call("pc_lanczos_nblocks", self.n, C.byref(nbl))
+        self.lz_nblocks = nbl.value
+        self.scratch = Scratch()
         self.S = torch.empty((kmax, kmax), dtype=F64, device="cuda")
         self.G = torch.empty((kmax, kmax), dtype=F64, device="cuda")
         self.stats = {"filter_steps": 0, "rr": 0}
@@ -70,28 +75,27 @@ class Eigensolver:
         return float(self.vec.dots(a, 1, b).item())
 
     def lanczos_bound(self, m: int = 6, nev: int = 6) -> float:
+        """Upper bound of the spectrum (lanczosForChefsi1.py:5-99): m Lanczos
+        steps from a random start, max |Ritz value| of T_m plus the last beta.
+        alpha and beta stay on the device (pc_lanczos_step); the host reads them
+        once at the end."""
         v0, v1, w = self.lz[0:1], self.lz[1:2], self.lz[2:3]
         rand_block(v1, self.n, 1, self.seeds.next(), normal=True)
-        nrm = math.sqrt(self._dot(v1, v1))
-        self.vec.lincomb(v1, alpha=1.0 / nrm)
+        coef = self.scratch.get("lz_coef", 2 * m + 2)
+        part = self.scratch.get("lz_part", 2 * self.lz_nblocks)
+        _lib.call("pc_lanczos_start", self.n, ptr(v1), ptr(part), stream())
         v0.zero_()
-        bet = 0.0
-        tmat = np.zeros((m + 1, m + 1))
-        bound = None
-        for k in range(1, m + 1):
-            self._h64_step(v1, v0, w, -bet)
-            alp = self._dot(v1, w)
-            self.vec.lincomb(w, alpha=1.0, x=v1, beta=-alp)
-            bet = math.sqrt(self._dot(w, w))
+        for k in range(m):
+            self._h64_step(v1, v0, w, 0.0)               # w = H v1
+            _lib.call("pc_lanczos_step", self.n, ptr(v0), ptr(v1), ptr(w), ptr(coef), k, ptr(part), stream())
             v0, v1, w = v1, w, v0
-            self.vec.lincomb(v1, alpha=1.0 / bet)
-            tmat[k - 1, k - 1] = alp
-            tmat[k, k - 1] = bet
-            tmat[k - 1, k] = bet
-            ntest = min(5 * nev, m)
-            if (k >= ntest and k % 10 == 0) or k == m:
-                rr = np.sort(np.linalg.eigvalsh(tmat[:k, :k]))
-                bound = float(np.max(np.abs(rr)) + bet)
+        c = coef.cpu().numpy()
+        alpha, beta = c[0:2 * m:2], c[1:2 * m:2]
+        tmat = np.diag(alpha) + np.diag(beta[:m - 1], 1) + np.diag(beta[:m - 1], -1)
+        rr = np.linalg.eigvalsh(tmat)
+        bound = float(np.max(np.abs(rr)) + beta[m - 1])
+        if not np.isfinite(bound):
+            raise _lib.ParsecHipError("lanczos_bound: non-finite bound (invariant subspace)")
         return bound
 
     def lancz_uppbnd(self, k: int = 6):

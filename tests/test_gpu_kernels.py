@@ -417,3 +417,77 @@ def test_rand_matches_mirror(gpu):
     assert np.array_equal(out[:, :n].cpu().numpy(), block(12345, n, k, False).T)
     rand_block(out, n, k, 777, normal=True)
     assert np.allclose(out[:, :n].cpu().numpy(), block(777, n, k, True).T, rtol=1e-14, atol=1e-14)
+
+
+@pytest.mark.parametrize("n", [1, 3000, 70001])
+def test_lanczos_step_vs_numpy(gpu, n):
+    """pc_lanczos_start / pc_lanczos_step: the three-term recurrence with alpha and
+    beta kept on the device equals NumPy's on the same start vector, given
+    w = M v1 from the host each step (M symmetric)."""
+    import ctypes as C
+    import torch
+    from parsec_python_amd import _lib
+    from parsec_python_amd.device import ptr, stream
+    print(f"lanczos step n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    m = 6 if n > 1 else 1
+    if n <= 3000:
+        Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+        lam = rng.uniform(-3.0, 5.0, n)
+        matvec = lambda x: Q @ (lam * (Q.T @ x))
+    else:
+        d = rng.uniform(-3.0, 5.0, n)          # a diagonal operator keeps the big case cheap
+        matvec = lambda x: d * x
+    v = rng.standard_normal(n)
+    nb = C.c_int()
+    _lib.call("pc_lanczos_nblocks", n, C.byref(nb))
+    part = torch.empty(2 * nb.value, dtype=torch.float64, device="cuda")
+    coef = torch.zeros(2 * m + 2, dtype=torch.float64, device="cuda")
+    v0 = torch.zeros(n, dtype=torch.float64, device="cuda")
+    v1 = torch.tensor(v, device="cuda")
+    w = torch.empty(n, dtype=torch.float64, device="cuda")
+    _lib.call("pc_lanczos_start", n, ptr(v1), ptr(part), stream())
+    # NumPy reference with the same arithmetic: w = M v1; alpha = v1.w; w -= alpha v1 + beta v0
+    r1 = v / np.linalg.norm(v)
+    r0 = np.zeros(n)
+    beta = 0.0
+    ref = []
+    for k in range(m):
+        w.copy_(torch.tensor(matvec(v1.cpu().numpy()), device="cuda"))
+        _lib.call("pc_lanczos_step", n, ptr(v0), ptr(v1), ptr(w), ptr(coef), k, ptr(part), stream())
+        v0, v1, w = v1, w, v0
+        rw = matvec(r1)
+        alpha = r1 @ rw
+        rw = rw - alpha * r1 - beta * r0
+        beta = np.linalg.norm(rw)
+        ref += [alpha, beta]
+        r0, r1 = r1, rw / beta if beta > 0 else rw
+        if n == 1:
+            break
+    got = coef.cpu().numpy()[:len(ref)]
+    if n == 1:
+        assert abs(got[0] - ref[0]) <= 1e-12 * max(1.0, abs(ref[0]))
+        return
+    assert np.allclose(got, ref, rtol=1e-10, atol=1e-12)
+
+
+def test_lanczos_bound_is_upper_bound(gpu):
+    """Eigensolver.lanczos_bound (device-side Lanczos) bounds the spectrum of H
+    from above, within 20 % of the largest eigenvalue (oracle sparse H)."""
+    import scipy.sparse as sps
+    import scipy.sparse.linalg as spla
+    import torch
+    from parsec_python_amd.eigensolvers import Eigensolver
+    from parsec_python_amd.hamiltonian import Hamiltonian
+    h = 0.4
+    g = _grid(22, 22, 22, h, "sphere")
+    n = g.npts
+    V = np.random.default_rng(3).uniform(-1.0, 0.5, n)
+    H = Hamiltonian(g, 4, None, torch.tensor(V, device="cuda"), torch.float64)
+    eig = Eigensolver(H, 8, seed=5)
+    bound = eig.lanczos_bound(6, 6)
+    A = (0.5 / h ** 2) * _oracle_lap(g, 4) + sps.diags(V)
+    lmax = spla.eigsh(A.tocsc(), k=1, which="LA", return_eigenvectors=False)[0]
+    assert lmax <= bound <= 1.2 * lmax, (bound, lmax)
+    eig.close()
+    H.close()
