@@ -54,7 +54,8 @@ int pc_stencil_num_partials(void* grid, int dtype_bytes, int nvec, int* out_coun
  * replaces: src/V_ion/pseudoNL_original.py:9-208 (V_nl as dense outer
  * products in a sparse matrix, applied inside every SpMM).
  * atom_ptr[natoms+1] groups entries; ent_pt[nent] point indices; ent_u[3*nent]
- * (u_x,u_y,u_z); scale[natoms] = h^3/Zvalue.  C is FP64 (3*natoms) x nvec. */
+ * (u_x,u_y,u_z); scale[natoms] = h^3/Zvalue.  C is FP64, atom-major:
+ * C[(a*nvec + v)*3 + m], natoms*nvec*3 doubles. */
 int pc_proj_create(int natoms, const int* atom_ptr, const int* ent_pt, const double* ent_u, const double* scale,
                    long long npts, void** out_proj);
 int pc_proj_destroy(void* proj);

@@ -4,8 +4,10 @@
 // dense outer products in a sparse matrix (src/V_ion/pseudoNL_original.py:
 // 187-206).  Here each atom keeps its support points and the three u values,
 // and V_nl*Y is applied as
-//   gather : C[v, 3a+m] = sum_{p in supp(a)} u_am(p) * Y[p, v]          (FP64)
-//   scatter: Out[p, v] += alpha * sum_{(a,m) covering p} scale_a u_am(p) C[v, 3a+m]
+//   gather : C[a, v, m] = sum_{p in supp(a)} u_am(p) * Y[p, v]          (FP64)
+//   scatter: Out[p, v] += alpha * sum_{(a,m) covering p} scale_a u_am(p) C[a, v, m]
+// C is atom-major, C[(a*nvec + v)*3 + m], so the scatter's 8 vectors of one
+// entry read 24 consecutive doubles (two cache lines, not eight).
 // The scatter runs over unique support points (no atomics, deterministic).
 #include "pc_common.h"
 #include <algorithm>
@@ -28,7 +30,7 @@ struct ProjDev {
   int* upt_ent;    // [nent] entry ids grouped by point
 };
 
-// C[v][3a + m] = sum_e u_m[e] * Y[v][pt[e]] (FP64 accumulation).  Workgroup =
+// C[a][v][m] = sum_e u_m[e] * Y[v][pt[e]] (FP64 accumulation).  Workgroup =
 // one atom x 16 vectors; each wavefront walks the atom's entries once and
 // applies them to 4 vectors, so the entry tables are read once per 4 vectors.
 constexpr int kGV = 4;   // vectors per wavefront in the gather
@@ -64,14 +66,14 @@ __global__ __launch_bounds__(256) void proj_gather(ProjDev P, const T* __restric
 #pragma unroll
     for (int u = 0; u < kGV; ++u) {
       if (v0 + u >= nvec) break;
-      double* cv = C + (long long)(v0 + u) * 3 * P.natoms + 3 * a;
+      double* cv = C + ((long long)a * nvec + (v0 + u)) * 3;
       cv[0] = s[u][0]; cv[1] = s[u][1]; cv[2] = s[u][2];
     }
   }
 }
 
 // Out[v][upt[q]] += alpha * sum over the entries e at point q of
-// scale[atom] * (u[e] . C[v][atom]).  One thread per unique point and 8
+// scale[atom] * (u[e] . C[atom][v]).  One thread per unique point and 8
 // vectors: the point's entry list is read once for the 8.
 constexpr int kSV = 8;   // vectors per thread in the scatter
 constexpr int kMaxEnt = 8;
@@ -96,7 +98,7 @@ __global__ __launch_bounds__(256) void proj_scatter(ProjDev P, const double* __r
 #pragma unroll
       for (int u = 0; u < kSV; ++u) {
         const int v = min(v0 + u, nvec - 1);
-        const double* c = C + (long long)v * 3 * P.natoms + 3 * a;
+        const double* c = C + ((long long)a * nvec + v) * 3;
         acc[u] += u0 * c[0] + u1 * c[1] + u2 * c[2];
       }
     }
