@@ -357,10 +357,14 @@ def predict_start(net: UNet, grid, rho0_cell: torch.Tensor, nelec: float) -> tor
     return rho * (nelec / float(rho.sum().item()))
 
 
-def learned_start(prob, net: UNet, cg_maxit: int = 1000, cg_tol: float = 1e-10):
+def learned_start(prob, net: UNet, cg_maxit: int | None = None, cg_tol: float = 1e-5):
     """(rho1, V_H1) for run_scf(rho_init=..., hpot_init=...), following the ML
     start of src/V_ion/pseudoDiag_ML4Den_poisson.py:392-540: rho1 is the
-    clipped, renormalised network density; V_H1 = V_H0 + CG(A, 4 pi/h^3 (rho1 - rho0))."""
+    clipped, renormalised network density; V_H1 = V_H0 + CG(A, 4 pi/h^3 (rho1 - rho0))
+    from a zero start to the reference's relative tolerance 1e-5 (:532-538); the
+    iteration cap defaults to the problem's Hartree cap."""
+    if cg_maxit is None:
+        cg_maxit = int(getattr(prob.cfg, "pcg_maxit", 200))
     from .hartree import HartreeSolver
     rho1 = predict_start(net, prob.grid, prob.rho0, prob.nelec)
     h3 = prob.h ** 3

@@ -33,17 +33,19 @@ def test_learned_start_contract(gpu):
     atoms, dom, nev = M.prepare(M.alkane(4, np.random.default_rng(HELDOUT_SEED)), h_ang=TRAIN_H_ANG,
                                 radius_ang=TRAIN_RADIUS_ANG)
     prob = SCFProblem(atoms, dom, nev, SCFConfig(domain="cube", pcg_maxit=1000))
-    rho1, vh1 = learned_start(prob, net)
-    r = rho1.cpu().numpy()
-    assert r.min() >= 0.0
-    assert abs(r.sum() - prob.nelec) < 1e-9 * prob.nelec
-    # (-lap/h^2)(V_H1 - V_H0) = 4 pi/h^3 (rho1 - rho0) to the CG tolerance
     from oracle import fd as ofd
     g = prob.grid
     A = ofd.laplacian_masked(g.nx, g.ny, g.nz, 4, g.mask) * (1.0 / g.h ** 2)
-    dv = (vh1 - prob.hpot0).cpu().numpy()
-    rhs = 4 * math.pi / g.h ** 3 * (r - prob.rho0.cpu().numpy())
-    assert np.linalg.norm(A @ dv - rhs) <= 1e-8 * np.linalg.norm(rhs)
+    # the reference's tolerance (pseudoDiag_ML4Den_poisson.py:532-538) by default, and a tight solve
+    for tol, kw in ((1e-5, {}), (1e-10, {"cg_tol": 1e-10})):
+        rho1, vh1 = learned_start(prob, net, **kw)
+        r = rho1.cpu().numpy()
+        assert r.min() >= 0.0
+        assert abs(r.sum() - prob.nelec) < 1e-9 * prob.nelec
+        # (-lap/h^2)(V_H1 - V_H0) = 4 pi/h^3 (rho1 - rho0) to the CG tolerance (zero start: r0 = rhs)
+        dv = (vh1 - prob.hpot0).cpu().numpy()
+        rhs = 4 * math.pi / g.h ** 3 * (r - prob.rho0.cpu().numpy())
+        assert np.linalg.norm(A @ dv - rhs) <= 1.05 * tol * np.linalg.norm(rhs)
     prob.close()
 
 
