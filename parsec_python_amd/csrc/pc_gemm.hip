@@ -355,6 +355,119 @@ __global__ __launch_bounds__(256) void rotate_kernel(const T* __restrict__ X, lo
       }
 }
 
+// FP32 form with every output column in one workgroup, so X is read from
+// memory once: waves (pw, cw) = (point half, column half) of 128 points x
+// 32*NCH columns; a wave accumulates 64 points x NCH column tiles.  Its 64
+// points form 4 interleaved tiles (tile j = points 4 n + j), so each lane loads
+// the 4 consecutive points it feeds to the 4 tiles with one float4 per
+// 4-vector step and stores them back with one float4 per output row.  Q^T
+// comes through LDS in 16-vector slabs (double-buffered; the next slab is
+// fetched into registers while the MFMAs run).  Requires ky <= 32*NCH,
+// 16-byte aligned X/Z columns (ld % 4 == 0).
+template <int NCH>
+__global__ __launch_bounds__(256, 2) void rotate_wide_f32(const float* __restrict__ X, long long ldx,
+                                                       const float* __restrict__ Qt, float* __restrict__ Z,
+                                                       long long ldz, long long n, int kx, int ky, int nblocks,
+                                                       int nb8) {
+  constexpr int KS = 16;             // vectors per Q slab
+  constexpr int NC = 32 * NCH;       // columns per workgroup
+  constexpr int QPT = KS * NC / 256; // slab values per thread
+  __shared__ float qs[2][KS * NC];
+  const int L = (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
+  if (L >= nblocks) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int pw = w >> 1, cw = w & 1;
+  const int lm = lane & 15, lk = lane >> 4;
+  const long long pl = (long long)L * 128 + pw * 64 + 4 * lm;   // this lane's 4 points
+  const int c0 = cw * 16 * NCH;
+  v4f acc[NCH][4];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[c][j] = v4f{0, 0, 0, 0};
+  float qr[QPT];
+  auto qfetch = [&](int kb) {
+#pragma unroll
+    for (int t = 0; t < QPT; ++t) {
+      const int e = threadIdx.x + 256 * t;
+      const int kk = e / NC, col = e - kk * NC;
+      qr[t] = (kb + kk < kx && col < ky) ? Qt[(long long)(kb + kk) * ky + col] : 0.0f;
+    }
+  };
+  auto qstore = [&](int bf) {
+#pragma unroll
+    for (int t = 0; t < QPT; ++t) qs[bf][threadIdx.x + 256 * t] = qr[t];
+  };
+  const bool full = pl + 3 < n;
+  // X of one slab (4 steps x 4 points per lane), fetched one slab ahead
+  float xc[KS / 4][4], xn[KS / 4][4];
+  auto xfetch = [&](int kb, float (&xo)[KS / 4][4]) {
+#pragma unroll
+    for (int st = 0; st < KS / 4; ++st) {
+      const int kk = kb + 4 * st + lk;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xo[st][j] = 0.0f;
+      if (kk < kx) {
+        const float* xr = X + (long long)kk * ldx + pl;
+        if (full) {
+          const float4 t = *reinterpret_cast<const float4*>(xr);
+          xo[st][0] = t.x; xo[st][1] = t.y; xo[st][2] = t.z; xo[st][3] = t.w;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) xo[st][j] = pl + j < n ? xr[j] : 0.0f;
+        }
+      }
+    }
+  };
+  qfetch(0);
+  xfetch(0, xc);
+  qstore(0);
+  __syncthreads();
+  int bf = 0;
+  for (int kb = 0; kb < kx; kb += KS) {
+    const bool more = kb + KS < kx;
+    if (more) {
+      qfetch(kb + KS);
+      xfetch(kb + KS, xn);
+    }
+#pragma unroll
+    for (int st = 0; st < KS / 4; ++st) {
+      const float* qrow = &qs[bf][(4 * st + lk) * NC + c0 + lm];
+#pragma unroll
+      for (int c = 0; c < NCH; ++c) {
+        const float qa = qrow[16 * c];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[c][j] = Acc<float>::mfma(qa, xc[st][j], acc[c][j]);
+      }
+    }
+    if (more) {
+      qstore(bf ^ 1);
+#pragma unroll
+      for (int st = 0; st < KS / 4; ++st)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) xc[st][j] = xn[st][j];
+    }
+    __syncthreads();
+    bf ^= 1;
+  }
+  // lane (lm, lk) register r of tile (c, j): column c0 + 16c + 4lk + r, point pl + j
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = c0 + 16 * c + Acc<float>::row(lk, r);
+      if (col >= ky) continue;
+      float* zr = Z + (long long)col * ldz + pl;
+      if (full) {
+        *reinterpret_cast<float4*>(zr) = make_float4(acc[c][0][r], acc[c][1][r], acc[c][2][r], acc[c][3][r]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (pl + j < n) zr[j] = acc[c][j][r];
+      }
+    }
+}
+
 __global__ void convert_q_f32(const double* __restrict__ Q, int ldq, int kx, int ky, float* __restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= kx * ky) return;
@@ -446,6 +559,25 @@ int rotate_impl(const T* X, long long ldx, const double* Q, int ldq, T* Z, long 
   else
     hipLaunchKernelGGL(convert_q_f64, dim3((ne + 255) / 256), dim3(256), 0, st, Q, ldq, kx, ky, qt);
   PC_LAUNCH_CHECK();
+  if constexpr (sizeof(T) == 4) {
+    const bool vec = (ldx % 4 == 0) && (ldz % 4 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0) &&
+                     (reinterpret_cast<uintptr_t>(Z) % 16 == 0);
+    const int nch = (ky + 31) / 32;
+    if (vec && nch <= 9) {
+      const int nb = (int)((n + 127) / 128);
+      const int nb8 = (nb + 7) / 8;
+      switch (nch) {
+#define PC_ROT(N)                                                                                                 \
+  case N:                                                                                                         \
+    hipLaunchKernelGGL(rotate_wide_f32<N>, dim3(8 * nb8), dim3(256), 0, st, X, ldx, qt, Z, ldz, n, kx, ky, nb, nb8); \
+    break;
+        PC_ROT(1) PC_ROT(2) PC_ROT(3) PC_ROT(4) PC_ROT(5) PC_ROT(6) PC_ROT(7) PC_ROT(8) PC_ROT(9)
+#undef PC_ROT
+      }
+      PC_LAUNCH_CHECK();
+      return pc::PC_OK;
+    }
+  }
   const int npt = (int)((n + 127) / 128);
   const int nct = (ky + 63) / 64;
   const int nb = npt * nct;
