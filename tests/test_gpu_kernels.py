@@ -140,8 +140,11 @@ def test_projectors_vs_oracle(gpu):
 
 
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("n,k", [(1000, 23), (5003, 49), (20000, 70)])
+@pytest.mark.parametrize("n,k", [(1000, 23), (5003, 49), (20000, 70), (513, 1), (3001, 275), (1201, 300)])
 def test_gram_and_rotate(gpu, dtype, n, k):
+    """S = X^T X, G = X^T Y and Z = X Q against NumPy.  The cases cover one to
+    nine 32-column groups of the all-columns FP32 rotation (k + 3 <= 288), its
+    fallback above that, ragged point counts and partial 16-vector slabs."""
     import torch
     from parsec_python_amd.hamiltonian import BlockLinalg, leading_dim
     print(f"gram case n={n} k={k} {dtype}", flush=True)
