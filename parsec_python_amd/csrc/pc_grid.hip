@@ -80,11 +80,10 @@ __device__ __forceinline__ T meta_val(const T* __restrict__ y, int2 m, int i) {
 // by PD).  A workgroup handles NV vectors at once: the row metadata, the V
 // values and the control flow are shared by them.  Two LDS tiles alternate,
 // so one barrier per plane suffices.
-template <typename T, int R, int SX, int TYW, int NV, int NB, bool HASX, bool DOT, int PD = 1>
-__global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
-                                                      T* Out, const T* __restrict__ V, T ca, T cb, T cc,
-                                                      double* __restrict__ dot_part) {
-  constexpr int NW = 4;
+template <typename T, int R, int SX, int TYW, int NV, int NB, bool HASX, bool DOT, int PD = 1, int NW = 4>
+__global__ __launch_bounds__(64 * NW) void stencil_zmarch(StArgs g, const T* __restrict__ Y, const T* X,
+                                                          T* Out, const T* __restrict__ V, T ca, T cb, T cc,
+                                                          double* __restrict__ dot_part) {
   constexpr int TY = NW * TYW;
   constexpr int NXP = 64 * SX;
   constexpr int LW = NXP + 2 * R;   // LDS row stride (R zero columns each side)
@@ -137,7 +136,7 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
     }
     return;
   }
-  for (int t = threadIdx.x; t < NB * NV * LR * LW; t += 256) (&tile[0][0][0])[t] = T(0);
+  for (int t = threadIdx.x; t < NB * NV * LR * LW; t += 64 * NW) (&tile[0][0][0])[t] = T(0);
   __syncthreads();
 
   auto rowmeta = [&](int j, int z) -> int2 {
@@ -333,19 +332,15 @@ __global__ __launch_bounds__(256) void stencil_zmarch(StArgs g, const T* __restr
   }
 }
 
-template <typename T, int R, int SX, int TYW, int NV, int NB = 2>
+template <typename T, int R, int SX, int TYW, int NV, int NB = 2, int NW = 4>
 constexpr int zmarch_lds_bytes() {
-  return NB * NV * (4 * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
+  return NB * NV * (NW * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
 }
 
-template <typename T, int R, int SX, int TYW, int NV, int NB = 2, int PD = 1>
+template <typename T, int R, int SX, int TYW, int NV, int NB = 2, int PD = 1, int NW = 4>
 int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                double ca, double cb, double cc, double* dot_part, hipStream_t st) {
-  if constexpr (NV > 1 && zmarch_lds_bytes<T, R, SX, TYW, NV, NB>() > 112 * 1024) {
-    // too much LDS for several vectors per workgroup: one vector per workgroup
-    return launch_cfg<T, R, SX, TYW, 1, NB, PD>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-  } else {
-  constexpr int TY = 4 * TYW;
+  constexpr int TY = NW * TYW;
   StArgs a;
   a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
   a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
@@ -357,30 +352,33 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
   for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
-  const dim3 grid(8 * a.nb8);
+  const dim3 grid(8 * a.nb8), block(64 * NW);
   if (dot_part) {
     if constexpr (NV == 1) {
       if (X)
-        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, true, PD>), grid, dim3(256), 0, st, a, Y, X, Out,
-                           V, T(ca), T(cb), T(cc), dot_part);
+        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, true, PD, NW>), grid, block, 0, st, a, Y, X,
+                           Out, V, T(ca), T(cb), T(cc), dot_part);
       else
-        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, true, PD>), grid, dim3(256), 0, st, a, Y, X,
+        hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, true, PD, NW>), grid, block, 0, st, a, Y, X,
                            Out, V, T(ca), T(cb), T(cc), dot_part);
     } else {
       pc::set_error("stencil: partial sums need one vector per workgroup");
       return pc::PC_ERR_ARG;
     }
   } else if (X) {
-    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, false, PD>), grid, dim3(256), 0, st, a, Y, X, Out, V,
-                       T(ca), T(cb), T(cc), dot_part);
+    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, true, false, PD, NW>), grid, block, 0, st, a, Y, X, Out,
+                       V, T(ca), T(cb), T(cc), dot_part);
   } else {
-    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, false, PD>), grid, dim3(256), 0, st, a, Y, X, Out,
+    hipLaunchKernelGGL((stencil_zmarch<T, R, SX, TYW, NV, NB, false, false, PD, NW>), grid, block, 0, st, a, Y, X, Out,
                        V, T(ca), T(cb), T(cc), dot_part);
   }
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
-  }
 }
+
+#ifndef PC_STENCIL_NW
+#define PC_STENCIL_NW 8
+#endif
 
 // One y-row per wavefront (TYW = 1), two vectors per workgroup where the two
 // LDS tiles fit in 56 KB (FP32 and FP64 up to radius 6 at nx <= 128), else one.
@@ -393,12 +391,15 @@ template <typename T, int R, int SX>
 int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                 double ca, double cb, double cc, double* dot_part, hipStream_t st) {
   constexpr int kLds = 56 * 1024;
+  constexpr int NW = PC_STENCIL_NW;
   if (dot_part)   // per-vector partial sums need one vector per workgroup
-    return launch_cfg<T, R, SX, 1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-  if constexpr (zmarch_lds_bytes<T, R, SX, 1, 2>() <= kLds)
-    return launch_cfg<T, R, SX, 1, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    return launch_cfg<T, R, SX, 1, 1, 2, 1, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  if constexpr (zmarch_lds_bytes<T, R, SX, 1, 2, 2, NW>() <= kLds)
+    return launch_cfg<T, R, SX, 1, 2, 2, 1, NW>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+  else if constexpr (zmarch_lds_bytes<T, R, SX, 1, 1, 2, NW>() <= kLds)
+    return launch_cfg<T, R, SX, 1, 1, 2, 1, NW>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
   else
-    return launch_cfg<T, R, SX, 1, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+    return launch_cfg<T, R, SX, 1, 1, 2, 1, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
 }
 
 template <typename T, int R>
