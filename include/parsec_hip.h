@@ -104,8 +104,15 @@ int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
 int pc_dense_factor_async(void* ctx, int k, double* S, int lds);
 /* eigensolver used by pc_dense_rr: 0 = rocSOLVER divide and conquer (syevd),
  * 1 = Jacobi (syevj), 2 = one-launch Householder tridiagonalization + rocSOLVER
- * stedc + ormtr (default; k <= 512, larger k uses syevd) */
+ * stedc + ormtr (default; 16 workgroups exchanging through agent-scope atomics
+ * for 128 <= k <= 320, one workgroup up to 512, larger k uses syevd),
+ * 3 = as 2 with the one-workgroup reduction always */
 int pc_dense_set_method(void* ctx, int method);
+/* test control of the multi-workgroup reduction: spins (> 0) = polls of one
+ * wait before it gives up and pc_dense_rr repeats the reduction with one
+ * workgroup; reset = 1 re-enables it; out_aborts = reductions repeated so far.
+ * No counterpart in the reference. */
+int pc_dense_trd_control(void* ctx, int spins, int reset, int* out_aborts);
 
 /* ---- point-wise SCF kernels --------------------------------------------------
  * density: rho = sum_v occ2[v] W_v^2           (src/rsdft_solver.py:324)

@@ -36,6 +36,11 @@ struct DenseCtx {
   int* info2;            // 4: potrf, trtri info of the side-stream factor, 2 unused
   const double* pend_S;  // factor queued for this S (nullptr: none)
   int pend_k, pend_lds;
+  int* sync;             // sytrd_lower_mwg: flags and abort word
+  double* pub;           // sytrd_lower_mwg: published p and column, [4][320]
+  bool mwg_ok;           // false after a multi-workgroup reduction gave up waiting
+  int mwg_aborts;        // reductions repeated with the one-workgroup kernel
+  int trd_spin;          // polls of one wait before sytrd_lower_mwg gives up
 };
 
 __global__ void symmetrize(double* G, int k, int ldg) {
@@ -55,7 +60,7 @@ __global__ void zero_info(int* info) {
 }
 
 __global__ void pack_out(const int* info, double* out, int k) {
-  if (threadIdx.x < 3) out[k + threadIdx.x] = (double)info[threadIdx.x];
+  if (threadIdx.x < 4) out[k + threadIdx.x] = (double)info[threadIdx.x];
 }
 
 // ---- one-workgroup Householder tridiagonalization (LAPACK dsytrd layout, lower)
@@ -255,6 +260,233 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_1wg(const double* __r
   if (n == 1 && tid == 0) d[0] = P[0];
 }
 
+// ---- multi-workgroup Householder tridiagonalization (same output as above)
+//
+// kTrdW workgroups of 256 threads; column c of the matrix lives in the LDS of
+// workgroup c % kTrdW for the whole reduction, so the trailing updates read and
+// write LDS instead of L2.  Step i:
+//   wave 0 of every workgroup (redundantly): w_{i-1} from the published
+//     p_{i-1}, then column i (published by its owner, updated through step
+//     i-2) brought up to date with (v_{i-1}, w_{i-1}) and reflector i built
+//     from it -- the same arithmetic in every workgroup, so no broadcast;
+//   all waves: each owned column c >= i+1 gets the pending rank-2 update of
+//     step i-1 and p_i(c) = column . v_i; p_i(c) and column i+1 are published;
+//   a barrier over the workgroups: each writes its step number to its own
+//     flag after a release fence, and polls the others' flags.
+// Published values, flags and the abort word are agent-scope atomics, so the
+// exchange does not depend on where the workgroups run.  A wait that exceeds
+// kTrdSpin polls (settable per context, for tests) sets the abort word and
+// every workgroup returns; the host then repeats the reduction with the
+// one-workgroup kernel.  All kTrdW workgroups must be resident at once: each
+// takes one CU's worth of LDS at most, far below the device's CU count, and the
+// kernel is the only work on its stream.
+constexpr int kTrdW = 16;                 // workgroups
+constexpr int kTrdMwgMax = 320;           // largest n (LDS: 20 columns of 320 rows)
+constexpr int kTrdMwgMin = 128;           // smaller n use the one-workgroup kernel
+constexpr int kTrdSpin = 1 << 21;         // polls before a wait is declared failed
+constexpr int kTrdFlagStride = 32;        // ints between two workgroups' flags
+constexpr int kTrdSyncInts = kTrdFlagStride * (kTrdW + 1);   // flags, then the abort word
+constexpr int kTrdLoc = (kTrdMwgMax + kTrdW - 1) / kTrdW;
+
+__device__ __forceinline__ void pub_store(double* p, double x) {
+  __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(x),
+                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double pub_load(const double* p) {
+  return __longlong_as_double((long long)__hip_atomic_load(reinterpret_cast<const unsigned long long*>(p),
+                                                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+
+template <int NCH>
+__global__ __launch_bounds__(256) void sytrd_lower_mwg(const double* __restrict__ G, int ldg, int n,
+                                                       double* __restrict__ P, double* __restrict__ d,
+                                                       double* __restrict__ e, double* __restrict__ tau,
+                                                       int* sync, double* pub, int* info_abort,
+                                                       int spin_limit) {
+  constexpr int LDP = 64 * NCH;
+  __shared__ double loc[kTrdLoc * LDP];                 // owned columns, rows 0..LDP
+  __shared__ double vp[LDP + 1], wp[LDP + 1], vc[LDP + 1], colsh[LDP];
+  __shared__ double tau_sh;
+  __shared__ int abort_sh;
+  const int wg = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int nloc = (n - wg + kTrdW - 1) / kTrdW;        // columns wg, wg + W, ... < n
+  int* flags = sync;
+  int* abort_word = sync + kTrdFlagStride * kTrdW;
+  double* pbuf = pub;                                   // [2][LDP]: p_i
+  double* cbuf = pub + 2 * LDP;                         // [2][LDP]: column i+1
+  for (int t = tid; t < kTrdLoc * LDP; t += 256) {
+    const int lc = t / LDP, r = t - lc * LDP;
+    const int c = wg + kTrdW * lc;
+    loc[t] = (lc < nloc && r < n) ? G[(long long)c * ldg + r] : 0.0;
+  }
+  for (int r = tid; r <= LDP; r += 256) {
+    vp[r] = 0.0;
+    wp[r] = 0.0;
+    vc[r] = 0.0;
+  }
+  if (tid == 0) abort_sh = 0;
+  __syncthreads();
+  for (int i = 0; i + 1 < n; ++i) {
+    // ---- wave 0: w_{i-1}, column i, reflector i
+    if (wv == 0) {
+      double x[NCH];
+      if (i > 0) {
+        const double* pb = pbuf + ((i - 1) & 1) * LDP;
+        const double* cb = cbuf + ((i - 1) & 1) * LDP;   // column i, published at step i-1
+        double pq[NCH], vq[NCH], dp = 0.0;
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+          const int r = lane + 64 * q;
+          pq[q] = (r >= i && r < n) ? pub_load(pb + r) : 0.0;
+          x[q] = r < n ? pub_load(cb + r) : 0.0;
+        }
+        const double t_prev = tau_sh;
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+          const int r = lane + 64 * q;
+          vq[q] = vc[r];
+          pq[q] *= t_prev;
+          dp += pq[q] * vq[q];
+        }
+        const double alpha2 = -0.5 * t_prev * wave_allsum(dp);
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+          const int r = lane + 64 * q;
+          const double wr = pq[q] + alpha2 * vq[q];
+          wp[r] = wr;
+          vp[r] = vq[q];
+        }
+        __builtin_amdgcn_wave_barrier();
+        const double wi = wp[i], vi = vp[i];
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+          const int r = lane + 64 * q;
+          x[q] -= vp[r] * wi + wp[r] * vi;
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < NCH; ++q) {
+          const int r = lane + 64 * q;
+          x[q] = r < n ? G[r] : 0.0;
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) colsh[lane + 64 * q] = x[q];
+      __builtin_amdgcn_wave_barrier();
+      const double dj = colsh[i], alpha = colsh[i + 1];
+      double sq = 0.0;
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        const int r = lane + 64 * q;
+        sq += (r >= i + 2 && r < n) ? x[q] * x[q] : 0.0;
+      }
+      const double xnorm2 = wave_allsum(sq);
+      double beta = alpha, t = 0.0, scal = 0.0;
+      if (xnorm2 > 0.0) {
+        const double nrm = sqrt(alpha * alpha + xnorm2);
+        beta = alpha >= 0.0 ? -nrm : nrm;
+        t = (beta - alpha) / beta;
+        scal = 1.0 / (alpha - beta);
+      }
+      double* __restrict__ pj = P + (long long)i * LDP;
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        const int r = lane + 64 * q;
+        const double out = r == i ? dj : (r == i + 1 ? beta : x[q] * scal);
+        vc[r] = (r <= i || r >= n) ? 0.0 : (r == i + 1 ? 1.0 : out);
+        if (wg == 0 && r >= i && r < n) pj[r] = out;
+      }
+      if (wg == 0 && lane == 0) {
+        d[i] = dj;
+        e[i] = beta;
+        tau[i] = t;
+      }
+      if (lane == 0) tau_sh = t;
+    }
+    __syncthreads();
+    // ---- all waves: pending update of step i-1 on the owned columns c >= i+1, p_i(c)
+    double* pb = pbuf + (i & 1) * LDP;
+    double* cb = cbuf + (i & 1) * LDP;
+    for (int lc = wv; lc < nloc; lc += 4) {
+      const int c = wg + kTrdW * lc;
+      if (c < i + 1) continue;
+      double* col = loc + lc * LDP;
+      const double vpc = vp[c], wpc = wp[c];
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < NCH; ++q) {
+        const int r = lane + 64 * q;
+        const double y = col[r] - (vp[r] * wpc + wp[r] * vpc);
+        col[r] = y;
+        acc += y * vc[r];
+        if (c == i + 1) pub_store(cb + r, y);
+      }
+      acc = wave_allsum(acc);
+      if (lane == 0) pub_store(pb + c, acc);
+    }
+    // ---- barrier over the workgroups
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (wv == 0) {
+      if (lane == 0) __hip_atomic_store(flags + kTrdFlagStride * wg, i + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      int spins = 0;
+      bool ok = false;
+      while (true) {
+        const int f = lane < kTrdW ? __hip_atomic_load(flags + kTrdFlagStride * lane, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) : (i + 1);
+        const int ab = __builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(abort_word, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        const bool all = __all(f >= i + 1);
+        if (ab != 0) break;
+        if (all) {
+          ok = true;
+          break;
+        }
+        if (++spins >= spin_limit) {
+          if (lane == 0) __hip_atomic_store(abort_word, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      if (lane == 0) abort_sh = ok ? 0 : 1;
+    }
+    __syncthreads();
+    if (abort_sh) {
+      if (wg == 0 && tid == 0) *info_abort = 1;
+      return;
+    }
+  }
+  // last diagonal entry: column n-1 through step n-2 (wave 0 of workgroup 0)
+  if (wg == 0 && wv == 0) {
+    const int i = n - 1;
+    const double* pb = pbuf + ((i - 1) & 1) * LDP;
+    const double* cb = cbuf + ((i - 1) & 1) * LDP;
+    const double t_prev = tau_sh;
+    double pq[NCH], vq[NCH], dp = 0.0;
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      const int r = lane + 64 * q;
+      pq[q] = (r >= i && r < n) ? t_prev * pub_load(pb + r) : 0.0;
+      vq[q] = vc[r];
+      dp += pq[q] * vq[q];
+    }
+    const double alpha2 = -0.5 * t_prev * wave_allsum(dp);
+    // only row n-1 is needed: x(n-1) - 2 v(n-1) w(n-1), with v(n-1) = 1
+    double dn = 0.0;
+#pragma unroll
+    for (int q = 0; q < NCH; ++q) {
+      const int r = lane + 64 * q;
+      if (r == i) {
+        const double wr = pq[q] + alpha2 * vq[q];
+        dn = pub_load(cb + r) - 2.0 * vq[q] * wr;
+        P[(long long)i * LDP + i] = dn;
+        d[i] = dn;
+      }
+    }
+  }
+}
+
 #define PC_RB(call)                                                                          \
   do {                                                                                       \
     rocblas_status s_ = (call);                                                              \
@@ -303,6 +535,11 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   rocblas_set_pointer_mode(c->h2, rocblas_pointer_mode_host);
   c->pend_S = nullptr;
   c->pend_k = c->pend_lds = 0;
+  PC_HIP(hipMalloc(&c->sync, kTrdSyncInts * sizeof(int)));
+  PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
+  c->mwg_ok = true;
+  c->mwg_aborts = 0;
+  c->trd_spin = kTrdSpin;
   c->method = 2;
   *out_ctx = c;
   return pc::PC_OK;
@@ -317,6 +554,8 @@ int pc_dense_destroy(void* ctx) {
   (void)hipEventDestroy(c->ev_in);
   (void)hipEventDestroy(c->ev_out);
   (void)hipFree(c->info2);
+  (void)hipFree(c->sync);
+  (void)hipFree(c->pub);
   rocblas_destroy_handle(c->h);
   (void)hipFree(c->tmp); (void)hipFree(c->E); (void)hipFree(c->out); (void)hipFree(c->info);
   (void)hipFree(c->resid); (void)hipFree(c->sweeps); (void)hipFree(c->tau); (void)hipFree(c->P);
@@ -363,41 +602,68 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
                       rocblas_diagonal_non_unit, k, k, &one, S, lds, c->tmp, k, Gy, ldg));
   hipLaunchKernelGGL(symmetrize, dim3((k * k + 255) / 256), dim3(256), 0, c->stream, Gy, k, ldg);
   PC_LAUNCH_CHECK();
-  const double* Q = Gy;   // eigenvectors of G
-  int ldq = ldg;
-  if (c->method == 2 && c->P && k <= kTrdMax) {
-    // G = H T H^T (one launch), T = Z diag(lam) Z^T (stedc), Q = H Z (ormtr)
-    const int nch = (k + 63) / 64;
-    switch (nch) {
+  for (int attempt = 0;; ++attempt) {
+    const double* Q = Gy;   // eigenvectors of G
+    int ldq = ldg;
+    bool mwg = false;
+    if ((c->method == 2 || c->method == 3) && c->P && k <= kTrdMax) {
+      // G = H T H^T, T = Z diag(lam) Z^T (stedc), Q = H Z (ormtr)
+      const int nch = (k + 63) / 64;
+      mwg = c->method == 2 && c->mwg_ok && k >= kTrdMwgMin && k <= kTrdMwgMax;
+      if (mwg) {
+        PC_HIP(hipMemsetAsync(c->sync, 0, kTrdSyncInts * sizeof(int), c->stream));
+        switch (nch) {
+#define PC_TRDM(N)                                                                                            \
+  case N:                                                                                                     \
+    hipLaunchKernelGGL(sytrd_lower_mwg<N>, dim3(kTrdW), dim3(256), 0, c->stream, Gy, ldg, k, c->P, c->out,   \
+                       c->E, c->tau, c->sync, c->pub, c->info + 3, c->trd_spin);                              \
+    break;
+          PC_TRDM(2) PC_TRDM(3) PC_TRDM(4) PC_TRDM(5)
+#undef PC_TRDM
+          default: break;
+        }
+      } else {
+        switch (nch) {
 #define PC_TRD(N)                                                                                                  \
   case N:                                                                                                          \
     hipLaunchKernelGGL(sytrd_lower_1wg<N>, dim3(1), dim3(kTrdThreads), 0, c->stream, Gy, ldg, k, c->P, c->out,  \
                        c->E, c->tau);                                                                              \
     break;
-      PC_TRD(1) PC_TRD(2) PC_TRD(3) PC_TRD(4) PC_TRD(5) PC_TRD(6) PC_TRD(7) PC_TRD(8)
+          PC_TRD(1) PC_TRD(2) PC_TRD(3) PC_TRD(4) PC_TRD(5) PC_TRD(6) PC_TRD(7) PC_TRD(8)
 #undef PC_TRD
+        }
+      }
+      PC_LAUNCH_CHECK();
+      PC_RB(rocsolver_dstedc(c->h, rocblas_evect_tridiagonal, k, c->out, c->E, c->tmp, k, c->info + 2));
+      PC_RB(rocsolver_dormtr(c->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, k, k, c->P,
+                             64 * nch, c->tau, c->tmp, k));
+      Q = c->tmp;
+      ldq = k;
+    } else if (c->method == 1) {
+      PC_RB(rocsolver_dsyevj(c->h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg,
+                             0.0, c->resid, 100, c->sweeps, c->out, c->info + 2));
+    } else {
+      PC_RB(rocsolver_dsyevd(c->h, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg, c->out, c->E,
+                             c->info + 2));
     }
+    // T = R^-1 Q
+    PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, k, k, &one, S, lds, Q, ldq, T, ldt));
+    PC_HIP(hipMemcpyAsync(lam_dev, c->out, k * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    hipLaunchKernelGGL(pack_out, dim3(1), dim3(64), 0, c->stream, c->info, c->out, k);
     PC_LAUNCH_CHECK();
-    PC_RB(rocsolver_dstedc(c->h, rocblas_evect_tridiagonal, k, c->out, c->E, c->tmp, k, c->info + 2));
-    PC_RB(rocsolver_dormtr(c->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, k, k, c->P, 64 * nch,
-                           c->tau, c->tmp, k));
-    Q = c->tmp;
-    ldq = k;
-  } else if (c->method == 1) {
-    PC_RB(rocsolver_dsyevj(c->h, rocblas_esort_ascending, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg,
-                           0.0, c->resid, 100, c->sweeps, c->out, c->info + 2));
-  } else {
-    PC_RB(rocsolver_dsyevd(c->h, rocblas_evect_original, rocblas_fill_upper, k, Gy, ldg, c->out, c->E,
-                           c->info + 2));
+    PC_HIP(hipMemcpyAsync(c->host_out, c->out, (k + 4) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    PC_HIP(hipStreamSynchronize(c->stream));
+    if (mwg && c->host_out[k + 3] != 0.0 && attempt == 0) {
+      // a workgroup of the multi-workgroup reduction gave up waiting: repeat
+      // with the one-workgroup kernel (G is unchanged) and keep using it
+      c->mwg_ok = false;
+      c->mwg_aborts += 1;
+      PC_HIP(hipMemsetAsync(c->info + 3, 0, sizeof(int), c->stream));
+      continue;
+    }
+    break;
   }
-  // T = R^-1 Q
-  PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                      rocblas_diagonal_non_unit, k, k, &one, S, lds, Q, ldq, T, ldt));
-  PC_HIP(hipMemcpyAsync(lam_dev, c->out, k * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-  hipLaunchKernelGGL(pack_out, dim3(1), dim3(64), 0, c->stream, c->info, c->out, k);
-  PC_LAUNCH_CHECK();
-  PC_HIP(hipMemcpyAsync(c->host_out, c->out, (k + 3) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  PC_HIP(hipStreamSynchronize(c->stream));
   const int i0 = (int)c->host_out[k], i1 = (int)c->host_out[k + 1], i2 = (int)c->host_out[k + 2];
   if (i0 != 0) {
     pc::set_error("pc_dense_rr: overlap matrix not positive definite (potrf info " + std::to_string(i0) + ")");
@@ -438,13 +704,32 @@ int pc_dense_factor_async(void* ctx, int k, double* S, int lds) {
 
 // Eigensolver of the projected k x k problem: 0 = divide and conquer
 // (rocSOLVER syevd), 1 = cyclic Jacobi (rocSOLVER syevj, tolerance = machine
-// precision, at most 100 sweeps), 2 = one-launch Householder tridiagonalization
-// (k <= 512; larger k uses syevd) + rocSOLVER stedc + ormtr.
+// precision, at most 100 sweeps), 2 = Householder tridiagonalization in one
+// launch (16 workgroups for 128 <= k <= 320, one workgroup otherwise; k > 512
+// uses syevd) + rocSOLVER stedc + ormtr, 3 = as 2 with one workgroup always.
 int pc_dense_set_method(void* ctx, int method) {
   DenseCtx* c = static_cast<DenseCtx*>(ctx);
   PC_CHECK_ARG(c, "null context");
-  PC_CHECK_ARG(method >= 0 && method <= 2, "method must be 0 (syevd), 1 (syevj) or 2 (tridiagonalization + stedc)");
+  PC_CHECK_ARG(method >= 0 && method <= 3,
+               "method must be 0 (syevd), 1 (syevj), 2 (tridiagonalization + stedc) or 3 (2, one workgroup)");
   c->method = method;
+  return pc::PC_OK;
+}
+
+// Polls of one inter-workgroup wait before the multi-workgroup
+// tridiagonalization gives up (spins >= 1; tests force the fallback with a
+// small value).  out_aborts (optional) receives the number of reductions this
+// context has repeated with the one-workgroup kernel; spins = 0 only reads it.
+int pc_dense_trd_control(void* ctx, int spins, int reset, int* out_aborts) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c, "null context");
+  PC_CHECK_ARG(spins >= 0, "spins < 0");
+  if (out_aborts) *out_aborts = c->mwg_aborts;
+  if (spins > 0) c->trd_spin = spins;
+  if (reset) {
+    c->mwg_ok = true;
+    c->mwg_aborts = 0;
+  }
   return pc::PC_OK;
 }
 

@@ -201,14 +201,15 @@ def test_dense_rr_generalized_eig(gpu):
     la.close()
 
 
-@pytest.mark.parametrize("method", [0, 1, 2])
-@pytest.mark.parametrize("k,kind", [(1, "rand"), (2, "rand"), (3, "rand"), (65, "rand"), (275, "degenerate"),
-                                    (300, "rand"), (520, "rand")])
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
+@pytest.mark.parametrize("k,kind", [(1, "rand"), (2, "rand"), (3, "rand"), (65, "rand"), (128, "rand"),
+                                    (275, "degenerate"), (300, "rand"), (320, "degenerate"), (520, "rand")])
 def test_dense_rr_methods(gpu, method, k, kind):
-    """Every eigensolver of the dense stage (syevd, syevj, one-launch
-    tridiagonalization + stedc) returns the generalized eigenpairs of (G, S):
-    T^T S T = I, T^T G T = diag(lam), lam = SciPy's.  'degenerate' has exact
-    triple eigenvalues, as the symmetric clusters give."""
+    """Every eigensolver of the dense stage (syevd, syevj, tridiagonalization
+    + stedc with 16 workgroups (method 2, 128 <= k <= 320) or one (method 3))
+    returns the generalized eigenpairs of (G, S): T^T S T = I, T^T G T =
+    diag(lam), lam = SciPy's.  'degenerate' has exact triple eigenvalues, as
+    the symmetric clusters give."""
     import torch
     from scipy.linalg import eigh
     from parsec_python_amd import _lib
@@ -239,6 +240,42 @@ def test_dense_rr_methods(gpu, method, k, kind):
     Tm = T.cpu().numpy()[:k * k].reshape(k, k).T  # column-major
     assert np.abs(Tm.T @ S @ Tm - np.eye(k)).max() < 1e-10 * k
     assert np.abs(Tm.T @ G @ Tm - np.diag(lam)).max() < 1e-10 * scale * k
+    aborts = C.c_int(-1)
+    _lib.call("pc_dense_trd_control", la.dense, 0, 0, C.byref(aborts))
+    assert aborts.value == 0          # the multi-workgroup waits all completed
+    la.close()
+
+
+@pytest.mark.parametrize("k", [128, 275])
+def test_dense_rr_mwg_fallback(gpu, k):
+    """With a one-poll limit on the multi-workgroup tridiagonalization's waits
+    (forced from the host; every workgroup still exits), pc_dense_rr repeats
+    the reduction with one workgroup and the result is unchanged."""
+    import torch
+    from scipy.linalg import eigh
+    from parsec_python_amd import _lib
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    print(f"dense_rr fallback k {k}", flush=True)
+    rng = np.random.default_rng(31 + k)
+    B = rng.standard_normal((2 * k, k))
+    S = B.T @ B / B.shape[0] + 0.5 * np.eye(k)
+    M = rng.standard_normal((k, k))
+    G = M + M.T
+    ref = eigh(G, S, eigvals_only=True)
+    la = BlockLinalg(10, k)
+    _lib.call("pc_dense_trd_control", la.dense, 1, 0, None)
+    aborts = C.c_int(-1)
+    for rep in range(2):
+        Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
+        Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
+        T, lam = la.dense_rr(Sd, Gd, k)
+        assert np.allclose(lam, ref, rtol=0, atol=1e-11 * max(1.0, np.abs(ref).max()) * k)
+        Tm = T.cpu().numpy()[:k * k].reshape(k, k).T
+        assert np.abs(Tm.T @ S @ Tm - np.eye(k)).max() < 1e-10 * k
+    _lib.call("pc_dense_trd_control", la.dense, 0, 0, C.byref(aborts))
+    print(f"  reductions repeated with one workgroup: {aborts.value}", flush=True)
+    assert aborts.value in (0, 1)     # at most once: the context then keeps one workgroup
+    _lib.call("pc_dense_trd_control", la.dense, 1 << 21, 1, None)
     la.close()
 
 
