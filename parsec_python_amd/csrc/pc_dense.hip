@@ -280,7 +280,10 @@ __global__ __launch_bounds__(kTrdThreads) void sytrd_lower_1wg(const double* __r
 // one-workgroup kernel.  All kTrdW workgroups must be resident at once: each
 // takes one CU's worth of LDS at most, far below the device's CU count, and the
 // kernel is the only work on its stream.
-constexpr int kTrdW = 16;                 // workgroups
+#ifndef PC_TRD_W
+#define PC_TRD_W 32
+#endif
+constexpr int kTrdW = PC_TRD_W;           // workgroups (at most 64: one poll lane each)
 constexpr int kTrdMwgMax = 320;           // largest n (LDS: 20 columns of 320 rows)
 constexpr int kTrdMwgMin = 128;           // smaller n use the one-workgroup kernel
 constexpr int kTrdSpin = 1 << 21;         // polls before a wait is declared failed
