@@ -104,7 +104,7 @@ int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
 int pc_dense_factor_async(void* ctx, int k, double* S, int lds);
 /* eigensolver used by pc_dense_rr: 0 = rocSOLVER divide and conquer (syevd),
  * 1 = Jacobi (syevj), 2 = one-launch Householder tridiagonalization + rocSOLVER
- * stedc + ormtr (default; 16 workgroups exchanging through agent-scope atomics
+ * stedc + ormtr (default; 32 workgroups exchanging through agent-scope atomics
  * for 128 <= k <= 320, one workgroup up to 512, larger k uses syevd),
  * 3 = as 2 with the one-workgroup reduction always */
 int pc_dense_set_method(void* ctx, int method);

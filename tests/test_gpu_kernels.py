@@ -206,7 +206,7 @@ def test_dense_rr_generalized_eig(gpu):
                                     (275, "degenerate"), (300, "rand"), (320, "degenerate"), (520, "rand")])
 def test_dense_rr_methods(gpu, method, k, kind):
     """Every eigensolver of the dense stage (syevd, syevj, tridiagonalization
-    + stedc with 16 workgroups (method 2, 128 <= k <= 320) or one (method 3))
+    + stedc with 32 workgroups (method 2, 128 <= k <= 320) or one (method 3))
     returns the generalized eigenpairs of (G, S): T^T S T = I, T^T G T =
     diag(lam), lam = SciPy's.  'degenerate' has exact triple eigenvalues, as
     the symmetric clusters give."""
