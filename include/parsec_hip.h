@@ -94,14 +94,6 @@ int pc_dense_destroy(void* ctx);
 int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, double* T, int ldt, double* lam_dev,
                 double* lam_host);
 int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
-/* pc_dense_factor_async: queues the potrf and trtri steps of pc_dense_rr for S
- * on the context's side stream (default priority), after the work already on
- * its stream, so that they overlap the caller's next kernels (H Y and its Gram
- * matrix); S is overwritten with R^-1.  The next pc_dense_rr with the same S,
- * k and lds waits for them and skips both steps.  No counterpart in the
- * reference (its QR runs after H W, chefsi1.py:43-64): a reordering of the
- * same arithmetic. */
-int pc_dense_factor_async(void* ctx, int k, double* S, int lds);
 /* eigensolver used by pc_dense_rr: 0 = rocSOLVER divide and conquer (syevd),
  * 1 = Jacobi (syevj), 2 = one-launch Householder tridiagonalization + rocSOLVER
  * stedc + ormtr (default; 32 workgroups exchanging through agent-scope atomics

@@ -56,7 +56,6 @@ PROTOTYPES = {
     "pc_dense_destroy": [_vp],
     "pc_dense_rr": [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _pd],
     "pc_dense_syevd": [_vp, _i, _vp, _i, _pd],
-    "pc_dense_factor_async": [_vp, _i, _vp, _i],
     "pc_dense_trd_control": [_vp, _i, _i, C.POINTER(_i)],
     "pc_dense_set_method": [_vp, _i],
     "pc_density_f64": [_vp, _ll, _ll, _i, _vp, _vp, _vp],

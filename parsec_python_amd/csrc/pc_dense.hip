@@ -29,13 +29,6 @@ struct DenseCtx {
   double* P;        // method 2: padded copy of G, 64*ceil(kmax/64) x (kmax+1)
   double* resid;    // syevj: off-diagonal norm at exit
   int* sweeps;      // syevj: sweeps used
-  // pc_dense_factor_async: Cholesky factor and inverse of S on a side stream
-  rocblas_handle h2;     // bound to `side`
-  hipStream_t side;      // default priority, non-blocking
-  hipEvent_t ev_in, ev_out;
-  int* info2;            // 4: potrf, trtri info of the side-stream factor, 2 unused
-  const double* pend_S;  // factor queued for this S (nullptr: none)
-  int pend_k, pend_lds;
   int* sync;             // sytrd_lower_mwg: flags and abort word
   double* pub;           // sytrd_lower_mwg: published p and column, [4][320]
   bool mwg_ok;           // false after a multi-workgroup reduction gave up waiting
@@ -527,18 +520,6 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   if (kmax <= kTrdMax)
     PC_HIP(hipMalloc(&c->P, (size_t)(64 * ((kmax + 63) / 64)) * (kmax + 1) * sizeof(double)));
   PC_HIP(hipHostMalloc(&c->host_out, (size_t)(kmax + 4) * sizeof(double), hipHostMallocDefault));
-  PC_HIP(hipMalloc(&c->info2, 4 * sizeof(int)));   // zero_info clears 4
-  PC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
-  PC_HIP(hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
-  PC_HIP(hipEventCreateWithFlags(&c->ev_out, hipEventDisableTiming));
-  if (rocblas_create_handle(&c->h2) != rocblas_status_success) {
-    pc::set_error("pc_dense_create: rocblas_create_handle failed");
-    return pc::PC_ERR_SOLVER;
-  }
-  rocblas_set_stream(c->h2, c->side);
-  rocblas_set_pointer_mode(c->h2, rocblas_pointer_mode_host);
-  c->pend_S = nullptr;
-  c->pend_k = c->pend_lds = 0;
   PC_HIP(hipMalloc(&c->sync, kTrdSyncInts * sizeof(int)));
   PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
   c->mwg_ok = true;
@@ -552,12 +533,6 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
 int pc_dense_destroy(void* ctx) {
   DenseCtx* c = static_cast<DenseCtx*>(ctx);
   if (!c) return pc::PC_OK;
-  (void)hipStreamSynchronize(c->side);
-  rocblas_destroy_handle(c->h2);
-  (void)hipStreamDestroy(c->side);
-  (void)hipEventDestroy(c->ev_in);
-  (void)hipEventDestroy(c->ev_out);
-  (void)hipFree(c->info2);
   (void)hipFree(c->sync);
   (void)hipFree(c->pub);
   rocblas_destroy_handle(c->h);
@@ -578,26 +553,10 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   PC_CHECK_ARG(k >= 1 && k <= c->kmax, "k out of range");
   PC_CHECK_ARG(lds >= k && ldg >= k && ldt >= k, "leading dimension too small");
   const double one = 1.0;
-  const bool queued = c->pend_S != nullptr;
-  const bool factored = queued && c->pend_S == S && c->pend_k == k && c->pend_lds == lds;
-  const bool clash = queued && !factored && c->pend_S == S;
-  c->pend_S = nullptr;
-  if (clash) {   // S already holds a factor of another size: waiting would not help
-    (void)hipStreamSynchronize(c->side);
-    pc::set_error("pc_dense_rr: S was queued to pc_dense_factor_async with another k or lds");
-    return pc::PC_ERR_ARG;
-  }
   hipLaunchKernelGGL(zero_info, dim3(1), dim3(64), 0, c->stream, c->info);
   PC_LAUNCH_CHECK();
-  if (queued) {   // the side stream's factor (of this S or another) completes first
-    PC_HIP(hipStreamWaitEvent(c->stream, c->ev_out, 0));
-  }
-  if (factored) {
-    PC_HIP(hipMemcpyAsync(c->info, c->info2, 2 * sizeof(int), hipMemcpyDeviceToDevice, c->stream));
-  } else {
-    PC_RB(rocsolver_dpotrf(c->h, rocblas_fill_upper, k, S, lds, c->info));
-    PC_RB(rocsolver_dtrtri(c->h, rocblas_fill_upper, rocblas_diagonal_non_unit, k, S, lds, c->info + 1));
-  }
+  PC_RB(rocsolver_dpotrf(c->h, rocblas_fill_upper, k, S, lds, c->info));
+  PC_RB(rocsolver_dtrtri(c->h, rocblas_fill_upper, rocblas_diagonal_non_unit, k, S, lds, c->info + 1));
   // tmp = Gy R^-1
   PC_RB(rocblas_dtrmm(c->h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                       rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg, c->tmp, k));
@@ -679,30 +638,6 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   }
   if (lam_host)
     for (int i = 0; i < k; ++i) lam_host[i] = c->host_out[i];
-  return pc::PC_OK;
-}
-
-// Queues S = R^T R (potrf) and S <- R^-1 (trtri) on the context's side stream,
-// after the work already queued on its stream, so that they overlap whatever
-// the caller queues next (H Y and its Gram matrix).  The next pc_dense_rr with
-// the same S, k and lds waits for them instead of repeating them; any other
-// pc_dense_rr still waits for them before it starts.
-int pc_dense_factor_async(void* ctx, int k, double* S, int lds) {
-  DenseCtx* c = static_cast<DenseCtx*>(ctx);
-  PC_CHECK_ARG(c && S, "null pointer");
-  PC_CHECK_ARG(k >= 1 && k <= c->kmax, "k out of range");
-  PC_CHECK_ARG(lds >= k, "leading dimension too small");
-  PC_CHECK_ARG(c->pend_S == nullptr, "a factor is already queued");
-  PC_HIP(hipEventRecord(c->ev_in, c->stream));
-  PC_HIP(hipStreamWaitEvent(c->side, c->ev_in, 0));
-  hipLaunchKernelGGL(zero_info, dim3(1), dim3(64), 0, c->side, c->info2);
-  PC_LAUNCH_CHECK();
-  PC_RB(rocsolver_dpotrf(c->h2, rocblas_fill_upper, k, S, lds, c->info2));
-  PC_RB(rocsolver_dtrtri(c->h2, rocblas_fill_upper, rocblas_diagonal_non_unit, k, S, lds, c->info2 + 1));
-  PC_HIP(hipEventRecord(c->ev_out, c->side));
-  c->pend_S = S;
-  c->pend_k = k;
-  c->pend_lds = lds;
   return pc::PC_OK;
 }
 

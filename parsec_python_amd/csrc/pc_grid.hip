@@ -379,6 +379,9 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
 #ifndef PC_STENCIL_NW
 #define PC_STENCIL_NW 8
 #endif
+#ifndef PC_STENCIL_NV
+#define PC_STENCIL_NV 1
+#endif
 
 // One y-row per wavefront (TYW = 1), two vectors per workgroup where the two
 // LDS tiles fit in 56 KB (FP32 and FP64 up to radius 6 at nx <= 128), else one.
@@ -394,7 +397,7 @@ int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
   constexpr int NW = PC_STENCIL_NW;
   if (dot_part)   // per-vector partial sums need one vector per workgroup
     return launch_cfg<T, R, SX, 1, 1, 2, 1, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
-  if constexpr (zmarch_lds_bytes<T, R, SX, 1, 2, 2, NW>() <= kLds)
+  if constexpr (PC_STENCIL_NV >= 2 && zmarch_lds_bytes<T, R, SX, 1, 2, 2, NW>() <= kLds)
     return launch_cfg<T, R, SX, 1, 2, 2, 1, NW>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
   else if constexpr (zmarch_lds_bytes<T, R, SX, 1, 1, 2, NW>() <= kLds)
     return launch_cfg<T, R, SX, 1, 1, 2, 1, NW>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);

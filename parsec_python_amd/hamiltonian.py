@@ -137,11 +137,6 @@ class BlockLinalg:
         _lib.call(f"pc_rotate_{suffix}", ptr(X), X.shape[1], ptr(Q), kx, ptr(out), out.shape[1], self.n, kx, ky,
                   ptr(work), kx * ky * esz, stream())
 
-    def factor_async(self, S: torch.Tensor, k: int) -> None:
-        """Queue the Cholesky factor of S (and its inverse, in place) on the
-        dense stage's side stream; the next dense_rr(S, ., k) uses it."""
-        _lib.call("pc_dense_factor_async", self.dense, k, ptr(S), k)
-
     def dense_rr(self, S: torch.Tensor, G: torch.Tensor, k: int):
         """Return (T (k x k column-major FP64 device), ritz values (host numpy))."""
         import numpy as np

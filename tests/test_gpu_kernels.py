@@ -279,44 +279,6 @@ def test_dense_rr_mwg_fallback(gpu, k):
     la.close()
 
 
-@pytest.mark.parametrize("k", [1, 65, 275])
-def test_dense_factor_async(gpu, k):
-    """The side-stream Cholesky factor gives the same T and Ritz values as
-    the in-line one (same rocSOLVER calls on another stream), with a stencil
-    launch queued in between, and its misuse raises."""
-    import torch
-    from parsec_python_amd import _lib
-    from parsec_python_amd.hamiltonian import BlockLinalg
-    print(f"dense factor async k {k}", flush=True)
-    rng = np.random.default_rng(11 + k)
-    B = rng.standard_normal((2 * k + 4, k))
-    S = B.T @ B / B.shape[0] + 0.5 * np.eye(k)
-    M = rng.standard_normal((k, k))
-    G = M + M.T
-    la = BlockLinalg(10, k)
-    Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
-    Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
-    T0, lam0 = la.dense_rr(Sd.clone(), Gd.clone(), k)
-    T0 = T0.clone()
-    S1, G1 = Sd.clone(), Gd.clone()
-    la.factor_async(S1, k)
-    filler = torch.randn(1 << 22, device="cuda", dtype=torch.float64)
-    filler.mul_(2.0)                          # main-stream work overlapping the factor
-    T1, lam1 = la.dense_rr(S1, G1, k)
-    assert np.allclose(lam0, lam1, rtol=0, atol=1e-12 * max(1.0, np.abs(lam0).max()))
-    assert torch.allclose(T0, T1[:k * k], rtol=0, atol=1e-12 * float(T0.abs().max()))
-    # a queued factor of S at another size cannot be used
-    if k > 1:
-        S2, G2 = Sd.clone(), Gd.clone()
-        la.factor_async(S2, k)
-        with pytest.raises(_lib.ParsecHipError):
-            la.dense_rr(S2, G2, k - 1)
-        # the context is usable again afterwards
-        T3, lam3 = la.dense_rr(Sd.clone(), Gd.clone(), k)
-        assert np.allclose(lam0, lam3, rtol=0, atol=1e-12 * max(1.0, np.abs(lam0).max()))
-    la.close()
-
-
 def test_density_xc_post_vs_oracle(gpu):
     import torch
     from parsec_python_amd import _lib
