@@ -383,13 +383,16 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
 #define PC_STENCIL_NV 1
 #endif
 
-// One y-row per wavefront (TYW = 1), two vectors per workgroup where the two
-// LDS tiles fit in 56 KB (FP32 and FP64 up to radius 6 at nx <= 128), else one.
-// Measured on Si87H76 (one recurrence step, 275 vectors) with the row metadata
-// in register windows: FP32 0.95 ms with 2 vectors, 0.96 ms with 1, 1.04 ms
-// with 4; FP64 1.72 ms with 2, 1.79 ms with 1.  A prefetch depth of two planes
-// (PD = 2) was slower (1.22 ms FP32, 4 vectors): the registers it takes cost
-// more occupancy than the extra plane in flight gains.
+// One y-row per wavefront (TYW = 1), PC_STENCIL_NW = 8 wavefronts (y-rows) and
+// PC_STENCIL_NV = 1 vector per workgroup where the LDS tiles fit in 56 KB, else
+// 4 wavefronts.  Measured on Si87H76 (one recurrence step, 275 vectors):
+//   4 rows: FP32 0.95 ms with 2 vectors, 0.96 ms with 1, 1.04 ms with 4;
+//           FP64 1.72 ms with 2, 1.79 ms with 1;
+//   8 rows: FP32 0.91 ms with 2 vectors (92 VGPRs, 5 waves per SIMD), 0.84 ms
+//           with 1 (60 VGPRs, 8 waves per SIMD); FP64 1.65 ms with 1;
+//   16 rows, 1 vector: FP32 0.84 ms, FP64 1.66 ms.
+// A prefetch depth of two planes (PD = 2) was slower (1.22 ms FP32, 4 vectors):
+// the registers it takes cost more occupancy than the extra plane gains.
 template <typename T, int R, int SX>
 int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                 double ca, double cb, double cc, double* dot_part, hipStream_t st) {
