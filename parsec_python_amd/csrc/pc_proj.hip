@@ -28,6 +28,8 @@ struct ProjDev {
   int* upt;        // [nupts] unique point index
   int* upt_ptr;    // [nupts+1] into upt_ent
   int* upt_ent;    // [nent] entry ids grouped by point
+  int* se_atom;    // [nent] atom of the t-th entry in point order
+  double* se_u;    // [3*nent] scale[atom] * u of the t-th entry in point order
 };
 
 // C[a][v][m] = sum_e u_m[e] * Y[v][pt[e]] (FP64 accumulation).  Workgroup =
@@ -74,9 +76,9 @@ __global__ __launch_bounds__(256) void proj_gather(ProjDev P, const T* __restric
 
 // Out[v][upt[q]] += alpha * sum over the entries e at point q of
 // scale[atom] * (u[e] . C[atom][v]).  One thread per unique point and 8
-// vectors: the point's entry list is read once for the 8.
+// vectors: the point's entry list (atom and scaled u, in point order, so one
+// dependent load instead of three) is read once for the 8.
 constexpr int kSV = 8;   // vectors per thread in the scatter
-constexpr int kMaxEnt = 8;
 template <typename T>
 __global__ __launch_bounds__(256) void proj_scatter(ProjDev P, const double* __restrict__ C, T* Out, long long ld,
                                                     int nvec, double alpha) {
@@ -85,29 +87,27 @@ __global__ __launch_bounds__(256) void proj_scatter(ProjDev P, const double* __r
   if (q >= P.nupts) return;
   const int t0 = P.upt_ptr[q], t1 = P.upt_ptr[q + 1];
   const long long pt = P.upt[q];
+  // the read half of the read-modify-write goes out with the entry loads
+  T ov[kSV];
+#pragma unroll
+  for (int u = 0; u < kSV; ++u) ov[u] = Out[(long long)min(v0 + u, nvec - 1) * ld + pt];
   double acc[kSV];
 #pragma unroll
   for (int u = 0; u < kSV; ++u) acc[u] = 0.0;
-  for (int tb = t0; tb < t1; tb += kMaxEnt) {
-    const int tn = min(kMaxEnt, t1 - tb);
-    for (int t = 0; t < tn; ++t) {
-      const int e = P.upt_ent[tb + t];
-      const int a = P.ent_atom[e];
-      const double sc = P.scale[a];
-      const double u0 = sc * P.ent_u[3 * e], u1 = sc * P.ent_u[3 * e + 1], u2 = sc * P.ent_u[3 * e + 2];
+  for (int t = t0; t < t1; ++t) {
+    const int a = P.se_atom[t];
+    const double u0 = P.se_u[3 * t], u1 = P.se_u[3 * t + 1], u2 = P.se_u[3 * t + 2];
 #pragma unroll
-      for (int u = 0; u < kSV; ++u) {
-        const int v = min(v0 + u, nvec - 1);
-        const double* c = C + ((long long)a * nvec + v) * 3;
-        acc[u] += u0 * c[0] + u1 * c[1] + u2 * c[2];
-      }
+    for (int u = 0; u < kSV; ++u) {
+      const int v = min(v0 + u, nvec - 1);
+      const double* c = C + ((long long)a * nvec + v) * 3;
+      acc[u] += u0 * c[0] + u1 * c[1] + u2 * c[2];
     }
   }
 #pragma unroll
   for (int u = 0; u < kSV; ++u) {
     if (v0 + u >= nvec) break;
-    T* o = Out + (long long)(v0 + u) * ld + pt;
-    *o = (T)((double)(*o) + alpha * acc[u]);
+    Out[(long long)(v0 + u) * ld + pt] = (T)((double)ov[u] + alpha * acc[u]);
   }
 }
 
@@ -179,6 +179,15 @@ int pc_proj_create(int natoms, const int* atom_ptr, const int* ent_pt, const dou
   alloc_copy((void**)&P->upt, upt.data(), upt.size() * sizeof(int));
   alloc_copy((void**)&P->upt_ptr, upt_ptr.data(), upt_ptr.size() * sizeof(int));
   alloc_copy((void**)&P->upt_ent, order.data(), nent * sizeof(int));
+  std::vector<int> se_atom(nent);
+  std::vector<double> se_u(3 * (size_t)nent);
+  for (int t = 0; t < nent; ++t) {
+    const int e = order[t], a = ent_atom[e];
+    se_atom[t] = a;
+    for (int m = 0; m < 3; ++m) se_u[3 * (size_t)t + m] = scale[a] * ent_u[3 * (size_t)e + m];
+  }
+  alloc_copy((void**)&P->se_atom, se_atom.data(), nent * sizeof(int));
+  alloc_copy((void**)&P->se_u, se_u.data(), 3 * (size_t)nent * sizeof(double));
   if (e != hipSuccess) {
     pc::set_error(std::string("pc_proj_create: ") + hipGetErrorString(e));
     return pc::PC_ERR_HIP;
@@ -192,6 +201,7 @@ int pc_proj_destroy(void* proj) {
   if (!P) return pc::PC_OK;
   (void)hipFree(P->atom_ptr); (void)hipFree(P->ent_pt); (void)hipFree(P->ent_u); (void)hipFree(P->ent_atom);
   (void)hipFree(P->scale); (void)hipFree(P->upt); (void)hipFree(P->upt_ptr); (void)hipFree(P->upt_ent);
+  (void)hipFree(P->se_atom); (void)hipFree(P->se_u);
   delete P;
   return pc::PC_OK;
 }
