@@ -65,6 +65,13 @@ class SCFConfig:
     polish: bool = True
     polish_de: float = 5e-6
     polish_err: float | None = None
+    # Chebyshev degree of the FP64 iterations after the switch (None: poldeg).
+    # By then the FP32 subspace is converged to FP32 accuracy and the FP64
+    # Rayleigh-Ritz is what fixes the energy; on Si87H76 (learned start) degree
+    # 2, 4, 6 and 10 all finish in 15 iterations, 3.4e-7, 1.8e-7, 4.3e-8 and
+    # 5.2e-7 Ha/atom from the converged FP64 energy, 0.430 / 0.434 / 0.438 /
+    # 0.446 s.
+    polish_deg: int | None = 4
     # eigensolver of the k x k Rayleigh-Ritz problem: "tridiag" (one-launch
     # Householder tridiagonalization + divide and conquer, k <= 512; larger k
     # falls back to "syevd"), "syevd" (rocSOLVER divide and conquer) or
@@ -221,7 +228,10 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
                     break
                 mean_prev = mean_now
         else:
-            W, lam = eig.chefsi1(W, lam, cfg.poldeg, nev)
+            deg = cfg.poldeg
+            if cfg.polish_deg is not None and H.dtype == F64 and prob.dtype == F32:
+                deg = cfg.polish_deg
+            W, lam = eig.chefsi1(W, lam, deg, nev)
         t1 = time.perf_counter()
         _, occ = occupations(lam[:nev], cfg.fermi_temp, prob.nelec, 1e-6)
         occ2_dev[:nev].copy_(torch.from_numpy(2.0 * np.asarray(occ, dtype=np.float64)))
