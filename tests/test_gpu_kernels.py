@@ -35,7 +35,7 @@ CASES = [  # nx, ny, nz, kind, radius
 
 @pytest.mark.parametrize("nx,ny,nz,kind,radius", CASES)
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
-@pytest.mark.parametrize("nvec", [2, 9])
+@pytest.mark.parametrize("nvec", [1, 2, 9])
 def test_stencil_step_vs_oracle(gpu, nx, ny, nz, kind, radius, dtype, nvec):
     import torch
     from parsec_python_amd.hamiltonian import Hamiltonian, leading_dim
