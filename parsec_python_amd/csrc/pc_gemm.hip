@@ -297,6 +297,8 @@ __global__ void gram_reduce(const double* __restrict__ part, int ntile, int nchu
     const int i = ti * 64 + ii, j = tj * 64 + jj;
     if (i >= kx || j >= ky) return;
     double s = 0.0;
+    // unrolled so that 16 loads per thread are in flight (the sum keeps its order)
+#pragma unroll 16
     for (int c = 0; c < nchunk; ++c) s += part[((long long)c * ntile + tile) * 4096 + e];
     S[(long long)j * lds + i] = s;
     if (sym && ti != tj) S[(long long)i * lds + j] = s;

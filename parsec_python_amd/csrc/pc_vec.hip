@@ -147,6 +147,7 @@ __global__ __launch_bounds__(kBlk) void lincomb_kernel(long long n, const double
   if (p >= n) return;
   double s = alpha * y[p];
   if (x) s += beta * x[p];
+#pragma unroll 8
   for (int i = 0; i < m; ++i) s += cf.c[i] * A[(long long)i * lda + p];
   y[p] = s;
 }
