@@ -174,7 +174,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the multi-GPU decomposition is designed, not built)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

@@ -453,8 +453,8 @@ def train_main(argv=None):
     ap.add_argument("--gen-budget-s", type=float, default=480.0, help="stop generating data after this long")
     ap.add_argument("--train-budget-s", type=float, default=300.0)
     ap.add_argument("--heldout", type=int, default=8)
-    ap.add_argument("--out", default="gpurun_out/unet_density.npz")
-    ap.add_argument("--report", default="gpurun_out/unet_training.json")
+    ap.add_argument("--out", default="unet_density.npz")
+    ap.add_argument("--report", default="unet_training.json")
     a = ap.parse_args(argv)
     rng = np.random.default_rng(a.seed)
     kinds = ["alkane"] * a.alkanes + ["si"] * a.si
