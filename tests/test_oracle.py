@@ -274,6 +274,66 @@ def test_msecant1_broyden_secant_equations_on_linear_problem():
     assert np.linalg.norm(f(x)) < np.linalg.norm(hist[0][1])
 
 
+def _dense_broyden1_inverse(B, dx, df):
+    """Broyden's first ("good") method in inverse form: the Sherman-Morrison
+    inverse of the rank-one Jacobian update, written for the iteration
+    x+ = x + B f (B ~ -J^-1, f the residual):
+    B+ = B - (dx + B df) dx^T B / (dx^T B df), which satisfies B+ df = -dx."""
+    Bdf = B @ df
+    return B - np.outer(dx + Bdf, dx @ B) / (dx @ Bdf)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_msecant1_is_dense_broyden1_and_meets_the_secant_equation(seed):
+    """Every update of the low-rank recursion (src/Mixer/msecant1.py:104-167,
+    group_size 1, EN_like 0) equals the dense inverse Broyden-I matrix built
+    independently from B0 = mix*I, and the stored pairs satisfy the secant
+    equation B_new df = -dx for the newest pair (to rounding)."""
+    rng = np.random.default_rng(seed)
+    n, mix = 15, 0.5
+    J = np.eye(n) + 0.3 * rng.standard_normal((n, n))          # nonlinear residual with this Jacobian at x*
+    xs = rng.standard_normal(n)
+    f = lambda x: -J @ (x - xs) - 0.05 * np.tanh(x - xs)      # noqa: E731
+    mx = Msecant1(mix=mix, restart_factor=0.0)               # no restarts: one uninterrupted secant history
+    B = mix * np.eye(n)
+    x = rng.standard_normal(n)
+    xp = fp = None
+    for it in range(8):
+        fx = f(x)
+        x_new, m = mx(x, fx)
+        assert m == it
+        if xp is not None:
+            dx, df = x - xp, fx - fp
+            B = _dense_broyden1_inverse(B, dx, df)
+            # the recursion's own B from its stored (DX, DF) pairs, the just-appended iterate excluded
+            Brec = mix * np.eye(n) - sum(np.outer(a, b) for a, b in zip(mx.DX[:-1], mx.DF[:-1]))
+            assert np.allclose(Brec, B, rtol=1e-10, atol=1e-12)
+            assert np.allclose(Brec @ df, -dx, rtol=1e-10, atol=1e-12)
+        assert np.allclose(x_new, x + B @ fx, rtol=1e-10, atol=1e-12)
+        xp, fp = x.copy(), fx.copy()
+        x = x_new
+
+
+def test_msecant1_mutation_is_caught():
+    """A plausible mistake (the sign of the N correction flipped) breaks the
+    secant equation, so the check above pins that term."""
+    rng = np.random.default_rng(5)
+    n = 10
+    dxs = [rng.standard_normal(n) for _ in range(3)]
+    dfs = [rng.standard_normal(n) for _ in range(3)]
+    mix = 0.5
+    B = mix * np.eye(n)
+    DX, DF = [], []
+    for dx, df in zip(dxs, dfs):
+        v = dx + mix * df - sum(a * (b @ df) for a, b in zip(DX, DF))
+        nv = -mix * dx - sum(b * (a @ dx) for a, b in zip(DX, DF))      # wrong sign on the history term
+        DX.append(v)
+        DF.append(nv / (nv @ df))
+        B = _dense_broyden1_inverse(B, dx, df)
+    Brec = mix * np.eye(n) - sum(np.outer(a, b) for a, b in zip(DX, DF))
+    assert not np.allclose(Brec, B, rtol=1e-6)
+
+
 def test_msecant1_restart_rule():
     mx = Msecant1()
     x0, f0 = np.ones(3), np.full(3, 1e-3)
