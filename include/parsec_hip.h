@@ -160,6 +160,17 @@ int pc_rand_f32(long long n, int ncol, long long ld, unsigned long long seed, in
 int pc_pcg_workspace(void* grid, size_t* out_bytes);
 int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
                  int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream);
+/* The same solve without any allocation, for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  The workspace is a fixed
+ * device array inside the library (64 MiB per device, held until the process
+ * ends), so the grid may hold at most pc_pcg_noalloc_max_points() points
+ * (2^21 = 128^3); larger grids return 4 (size limit).  Results are those of
+ * pc_pcg_solve, bit for bit.  Every call on one device shares that array:
+ * calls that overlap on one device (other host threads or streams) race. */
+int pc_pcg_solve_noalloc(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol,
+                         int alias_first, int check_every, int* its_out, double* res_out, void* stream);
+int pc_pcg_noalloc_max_points(long long* out);
 
 /* Multigrid-preconditioned CG on the same system and stop rule (replaces the
  * reference's optional preconditioned CG, cg_prec in src/Eigensolvers/pcg.py:7-85,

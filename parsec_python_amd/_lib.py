@@ -77,6 +77,8 @@ PROTOTYPES = {
     "pc_rand_f32": [_ll, _i, _ll, _u64, _i, _vp, _vp],
     "pc_pcg_workspace": [_vp, _psz],
     "pc_pcg_solve": [_vp, _d, _vp, _vp, _i, _d, _i, _i, _vp, _sz, _pi, _vp, _vp],
+    "pc_pcg_solve_noalloc": [_vp, _d, _vp, _vp, _i, _d, _i, _i, _pi, _vp, _vp],
+    "pc_pcg_noalloc_max_points": [_pll],
     "pc_mg_create": [_vp, _d, C.POINTER(_vp)],
     "pc_mg_destroy": [_vp],
     "pc_mg_workspace": [_vp, _psz],

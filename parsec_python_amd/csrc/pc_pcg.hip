@@ -22,6 +22,8 @@
 #include "pc_common.h"
 #include "pc_zmarch.h"
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 long long pc_internal_grid_npts(void* grid);
 
@@ -245,29 +247,14 @@ Launchers launchers(int sx) {
   return {launch_init<R>, launch_apply_z<R, 4>};
 }
 
-}  // namespace
-
-extern "C" {
-
-int pc_pcg_workspace(void* grid, size_t* out_bytes) {
-  PC_CHECK_ARG(grid && out_bytes, "null pointer");
-  *out_bytes = work_bytes_for(pc_internal_grid_npts(grid));
-  return pc::PC_OK;
-}
-
-// x (in: warm start, out: solution) and rhs are device FP64 arrays of npts.
-// inv_h2 = 1/h^2.  its_out receives the iteration count (host); res_out (host,
-// optional, 2 doubles) receives the final r.r and tol^2 * r0.r0.
-int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
-                 int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream) {
-  PC_CHECK_ARG(grid && rhs && x && work && its_out, "null pointer");
+// Shared body of the two solve entry points: `work` holds work_bytes_for(n)
+// bytes (8-byte aligned, no initial contents needed: every word is written
+// before it is read), `host` S_COUNT doubles of host memory for the state
+// read back every check_every iterations.
+int pcg_solve_impl(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
+                   int check_every, double* work, double* host, int* its_out, double* res_out, hipStream_t st) {
   const long long n = pc_internal_grid_npts(grid);
-  PC_CHECK_ARG(n > 0, "empty domain");
-  PC_CHECK_ARG(work_bytes >= work_bytes_for(n), "workspace too small");
-  PC_CHECK_ARG(((uintptr_t)work & 7) == 0, "workspace not 8-byte aligned");
-  PC_CHECK_ARG(maxit >= 0, "maxit < 0");
   if (check_every < 1) check_every = 8;
-  hipStream_t st = static_cast<hipStream_t>(stream);
 
   PointMap g;
   int radius = 0;
@@ -300,7 +287,7 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
   const int nb_launch = 8 * g.nb8;
 
   Work w;
-  w.r = static_cast<double*>(work);
+  w.r = work;
   w.p[0] = w.r + n;
   w.p[1] = w.p[0] + n;
   w.ap = w.p[1] + n;
@@ -313,9 +300,6 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
     pc::set_error("pc_pcg_solve: launch failed");
     return pc::PC_ERR_HIP;
   }
-  // pinned readback buffer, allocated once per host thread and kept
-  static thread_local double* host = nullptr;
-  if (!host) PC_HIP(hipHostMalloc(&host, S_COUNT * sizeof(double), hipHostMallocDefault));
   int status = pc::PC_OK;
   // iteration t: apply(t) decides convergence from r_t; update(t) makes r_{t+1}.
   // apply(maxit) only records the final state.
@@ -355,6 +339,98 @@ int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int ma
     }
   }
   return status;
+}
+
+// ---- the solve without allocations ----------------------------------------------
+// A file-scope device array replaces the caller workspace, sized for grids of at
+// most kNoAllocMaxPts points (128^3).  Every device holds its own copy at its own
+// address; host code never uses the array's name as a device address (it is a
+// host-side stand-in), only the pointer hipGetSymbolAddress returned for the
+// current device.
+constexpr long long kNoAllocMaxPts = 1LL << 21;
+constexpr size_t kNoAllocDoubles = (size_t)(4 * kNoAllocMaxPts + 2 * kMaxNB + kMaxWG + S_COUNT);
+
+}  // namespace
+
+// 64 MiB + 12 KiB of device memory per device the library runs on, held until
+// the process ends.
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, looked up once per device.
+int scratch_ptr(double** out) {
+  int dev = -1;
+  PC_HIP(hipGetDevice(&dev));
+  PC_CHECK_ARG(dev >= 0 && dev < kMaxDevices, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    PC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return pc::PC_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pc_pcg_workspace(void* grid, size_t* out_bytes) {
+  PC_CHECK_ARG(grid && out_bytes, "null pointer");
+  *out_bytes = work_bytes_for(pc_internal_grid_npts(grid));
+  return pc::PC_OK;
+}
+
+// x (in: warm start, out: solution) and rhs are device FP64 arrays of npts.
+// inv_h2 = 1/h^2.  its_out receives the iteration count (host); res_out (host,
+// optional, 2 doubles) receives the final r.r and tol^2 * r0.r0.
+int pc_pcg_solve(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol, int alias_first,
+                 int check_every, void* work, size_t work_bytes, int* its_out, double* res_out, void* stream) {
+  PC_CHECK_ARG(grid && rhs && x && work && its_out, "null pointer");
+  const long long n = pc_internal_grid_npts(grid);
+  PC_CHECK_ARG(n > 0, "empty domain");
+  PC_CHECK_ARG(work_bytes >= work_bytes_for(n), "workspace too small");
+  PC_CHECK_ARG(((uintptr_t)work & 7) == 0, "workspace not 8-byte aligned");
+  PC_CHECK_ARG(maxit >= 0, "maxit < 0");
+  // pinned readback buffer, allocated once per host thread and kept
+  static thread_local double* host = nullptr;
+  if (!host) PC_HIP(hipHostMalloc(&host, S_COUNT * sizeof(double), hipHostMallocDefault));
+  return pcg_solve_impl(grid, inv_h2, rhs, x, maxit, tol, alias_first, check_every, static_cast<double*>(work), host,
+                        its_out, res_out, static_cast<hipStream_t>(stream));
+}
+
+// The same solve with no allocation of any kind: the workspace is the library's
+// device array `scratch` (grids of at most 2^21 points) and the state is read
+// back into a stack buffer.  Calls on one device share that array, so calls
+// that overlap on one device (other host threads, other streams) race.
+int pc_pcg_solve_noalloc(void* grid, double inv_h2, const double* rhs, double* x, int maxit, double tol,
+                         int alias_first, int check_every, int* its_out, double* res_out, void* stream) {
+  PC_CHECK_ARG(grid && rhs && x && its_out, "null pointer");
+  const long long n = pc_internal_grid_npts(grid);
+  PC_CHECK_ARG(n > 0, "empty domain");
+  PC_CHECK_ARG(maxit >= 0, "maxit < 0");
+  if (n > kNoAllocMaxPts) {
+    pc::set_error("pc_pcg_solve_noalloc: grid of " + std::to_string(n) + " points exceeds the maximum of " +
+                  std::to_string(kNoAllocMaxPts) + " points of the library's fixed workspace");
+    return pc::PC_ERR_LIMIT;
+  }
+  double* work = nullptr;
+  if (int e = scratch_ptr(&work)) return e;
+  double host[S_COUNT];
+  return pcg_solve_impl(grid, inv_h2, rhs, x, maxit, tol, alias_first, check_every, work, host, its_out, res_out,
+                        static_cast<hipStream_t>(stream));
+}
+
+int pc_pcg_noalloc_max_points(long long* out) {
+  PC_CHECK_ARG(out, "null pointer");
+  *out = kNoAllocMaxPts;
+  return pc::PC_OK;
 }
 
 }  // extern "C"
