@@ -81,10 +81,13 @@ class SCFResult:
 
 def run_scf(setup: Setup, nev: int, *, tol=2e-4, maxits=40, poldeg=10, fermi_temp=500.0,
             seed=0, etol_ha_per_atom=None, max_wall=None, pcg_aliasing=True,
-            rho_init=None, hpot_init=None, rng=None) -> SCFResult:
+            rho_init=None, hpot_init=None, rng=None, xc_loop=False) -> SCFResult:
     """Run the reference SCF loop.  ``rho_init``/``hpot_init`` replace (rho0, V_H0)
     for a learned/ML start (pseudoDiag_ML4Den_poisson.py contract).  ``rng``
-    supplies ``random``/``standard_normal`` start vectors (default: PCG64(seed))."""
+    supplies ``random``/``standard_normal`` start vectors (default: PCG64(seed)).
+    ``xc_loop`` evaluates the XC point by point as the reference does (same
+    values; used where the reference's cost is timed)."""
+    exc_fn = xc.exc_nspn_loop if xc_loop else xc.exc_nspn
     if rng is None:
         rng = np.random.default_rng(seed)
     h = setup.domain["h"]
@@ -92,7 +95,7 @@ def run_scf(setup: Setup, nev: int, *, tol=2e-4, maxits=40, poldeg=10, fermi_tem
     rho0 = setup.rho0 if rho_init is None else rho_init
     hpot0 = setup.hpot0 if hpot_init is None else hpot_init
     hpot = np.zeros(n)
-    vxc, exc = xc.exc_nspn(h, rho0 / h ** 3)
+    vxc, exc = exc_fn(h, rho0 / h ** 3)
     pot = setup.ppot + hpot0 + 0.5 * vxc
     mixer = Msecant1()
     half_a_plus_vnl = (0.5 * setup.A + setup.vnl).tocsr()
@@ -121,7 +124,7 @@ def run_scf(setup: Setup, nev: int, *, tol=2e-4, maxits=40, poldeg=10, fermi_tem
         rho = rho / h ** 3
         hpot, cg_its = pcg(setup.A, hrhs, hpot, 200, 1e-5, reference_aliasing=pcg_aliasing)
         t2 = time.perf_counter()
-        vxc, exc = xc.exc_nspn(h, rho)
+        vxc, exc = exc_fn(h, rho)
         pot_new = setup.ppot + 0.5 * vxc + hpot + hpot0
         err = float(np.linalg.norm(pot_new - pot) / np.linalg.norm(pot_new))
         energies = energy_components(rho, hpot, hpot0, vxc, exc, lam, occ, nev, setup.e_nuc, h)

@@ -39,3 +39,30 @@ def exc_nspn(h: float, rho: np.ndarray):
     v[pos] = vx + vc
     e = e + rp * ec
     return v, float(np.sum(e)) * h ** 3
+
+
+def exc_nspn_loop(h: float, rho: np.ndarray):
+    """The same functional evaluated point by point in a Python loop, in the
+    reference's order (``src/V_xc/exc_nspn.py:73-91``, energy accumulated as
+    it goes).  Used where the reference's own cost is being timed."""
+    rho = np.asarray(rho, dtype=np.float64)
+    v = rho.copy()
+    exc = 0.0
+    sqrt, log = np.sqrt, np.log
+    for i in range(v.shape[0]):
+        r = v[i]
+        v[i] = 0.0
+        if r > 0.0:
+            rs = (P75VPI / r) ** THIRD
+            v[i] = -TWOVPIA0 / rs
+            exc += 0.75 * r * v[i]
+            if rs >= 1.0:
+                sq = sqrt(rs)
+                ec = G / (1.0 + B1 * sq + B2 * rs)
+                v[i] += (ec ** 2) * (1.0 + 3.5 * B1 * sq * THIRD + 4.0 * B2 * rs * THIRD) / G
+            else:
+                al = log(rs)
+                ec = C1 * al - C2 + (C3 * al - C4) * rs
+                v[i] += ec - (C1 + (C3 * al - C5) * rs) * THIRD
+            exc += r * ec
+    return v, exc * h ** 3

@@ -145,10 +145,17 @@ def _xc(n, rho, vxc, part, out1):
 
 
 def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: torch.Tensor | None = None,
-            max_wall: float | None = None, warmup: int | None = None) -> SCFResult:
+            max_wall: float | None = None, warmup: int | None = None,
+            start_weight_rho: torch.Tensor | None = None) -> SCFResult:
     """Run the SCF loop.  ``rho_init``/``hpot_init`` replace (rho0, V_H0) for a
     learned start, as the reference's ML density path does
     (``src/V_ion/pseudoDiag_ML4Den_poisson.py:392-540``).
+
+    ``start_weight_rho``: the density whose square root weights first_filt's
+    random start vectors (``cfg.weighted_start``); default ``rho_init`` when
+    it is given, else no weighting.  Passing ``prob.rho0`` gives the SAD start
+    with SAD-weighted start vectors, the control that separates the learned
+    density's share of the gain from the weighting's.
 
     ``warmup`` (default: ``cfg.start_warmup`` when ``rho_init`` is given, else
     0): up to that many extra filter + Rayleigh-Ritz passes at the starting
@@ -213,8 +220,9 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         H.set_potential(pot)
         if its == 1:
             sw = None
-            if rho_init is not None and cfg.weighted_start:
-                r = torch.clamp(rho_init, min=0.0)
+            wrho = start_weight_rho if start_weight_rho is not None else rho_init
+            if wrho is not None and cfg.weighted_start:
+                r = torch.clamp(wrho, min=0.0)
                 sw = torch.sqrt(r / r.max())
             W, lam = eig.first_filt(k, cfg.poldeg, start_weight=sw)
             first_filt_info = dict(eig.last_first_filt)

@@ -145,6 +145,15 @@ def test_xc_pz81_closed_form_and_continuity():
     assert v[2] == pytest.approx(vx + vc_hi, rel=1e-14)
 
 
+def test_xc_point_loop_equals_vectorised():
+    rng = np.random.default_rng(4)
+    rho = np.concatenate([rng.uniform(0, 2.0, 500), rng.uniform(0, 1e-4, 500), [0.0, -1e-3, 3 / (4 * np.pi)]])
+    v1, e1 = xc.exc_nspn(0.3, rho)
+    v2, e2 = xc.exc_nspn_loop(0.3, rho)
+    assert np.allclose(v1, v2, rtol=1e-14, atol=0)
+    assert e1 == pytest.approx(e2, rel=1e-13)
+
+
 # ------------------------------------------------------------------ projectors
 def test_projector_tables_match_pointwise_loop():
     """Factored tables == a direct per-point restatement of pseudoNL_original.py:143-201."""
@@ -356,3 +365,33 @@ def test_h2o_sample_scf_reproduces_reference_log():
     assert np.allclose(2 * r.eigenvalues[:4], log["occupied_2lambda_ry"], atol=5e-5)
     gold = np.load(os.path.join(GOLD, "h2o_sample_rho_grid.npz"))
     assert np.max(np.abs(r.rho * dom["h"] ** 3 - gold["conv"])) < 5e-6
+
+
+# ------------------------------------------------------------------ reference-generated vectors
+def test_pseudo_columns_and_vnl_match_reference_vectors():
+    """V_loc, SAD rho0 and V_H0 (src/V_ion/pseudoDiag.py:44-421) and V_nl @ X
+    (src/V_ion/pseudoNL_original.py:9-208) on a 12^3 H2O grid, against vectors
+    the reference's own CPU functions wrote (tests/golden/make_reference_scf.py)."""
+    d = np.load(os.path.join(GOLD, "reference_ionic_small.npz"))
+    nx = int(d["nx"])
+    dom = {"nx": nx, "ny": nx, "nz": nx, "h": float(d["h"]), "radius": float(d["radius"])}
+    coords = d["coords_bohr"]
+    species = [str(s) for s in d["species"]]
+    atoms = []
+    for sym in dict.fromkeys(species):
+        atoms.append((sym, coords[[i for i, s in enumerate(species) if s == sym]]))
+    # the same atoms as the product's own preparation of that input
+    atoms_p, dom_p, _ = M.prepare(M.h2o(), h_ang=0.5, radius_ang=3.0, nev=4)
+    assert dom_p["nx"] == nx and abs(dom_p["radius"] - dom["radius"]) < 1e-12
+    for (s1, c1), (s2, c2) in zip(atoms, atoms_p):
+        assert s1 == s2 and np.allclose(c1, c2, atol=1e-12)
+    mask = np.ones(nx ** 3, bool)
+    rho0, hpot0, pot, _ = pseudo.pseudo_diag(dom, atoms, mask)
+    assert np.max(np.abs(pot - d["ppot"])) <= 1e-12 * np.max(np.abs(d["ppot"]))
+    assert np.max(np.abs(rho0 - d["rho0"])) <= 1e-12 * np.max(np.abs(d["rho0"]))
+    assert np.max(np.abs(hpot0 - d["hpot0"])) <= 1e-12 * np.max(np.abs(d["hpot0"]))
+    vnl = pseudo.pseudo_nl(dom, atoms, mask)
+    vx = vnl @ d["X"]
+    assert np.max(np.abs(vx - d["VnlX"])) <= 1e-12 * np.max(np.abs(d["VnlX"]))
+    tabs = pseudo.projector_tables(dom, atoms, mask)
+    assert np.max(np.abs(pseudo.apply_nl(tabs, d["X"]) - d["VnlX"])) <= 1e-12 * np.max(np.abs(d["VnlX"]))
