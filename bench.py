@@ -7,9 +7,13 @@ precision configuration (FP32 filter / H*X / subspace GEMMs with FP64-
 accumulated Gram matrices; FP64 Cholesky, eigensolver, Hartree CG, mixer and
 energies) started from the learned density (the network's forward pass and the
 Hartree correction of its start density are inside the timed region; the
-weights are loaded before).  The plain FP64 solve from the SAD start is timed
-beside it, as is the mixed solve from the SAD start.  Accuracy is reported
-against a tightly converged FP64 solve (|dE| < 1e-9 Ha/atom, untimed).
+weights are loaded before).  Timed beside it, each as one warm-up solve and
+then the mean of several: the plain FP64 solve from the SAD start (the
+baseline of the metric), the same with the multigrid Hartree CG, the mixed
+solve from the SAD start, and the mixed solve from the SAD start with its
+random start vectors weighted by sqrt(rho_SAD) (the weighting the learned
+start uses; the control that isolates what the network adds).  Accuracy is
+reported against a tightly converged FP64 solve (|dE| < 1e-9 Ha/atom, untimed).
 
 Prints ONE JSON line (rank 0, N=1).  ``roofline`` is for the dominant kernel,
 the multi-vector fused stencil (HBM-bound): achieved = algorithmic bytes
@@ -19,8 +23,11 @@ HIP-event durations measured in the timed region on the launch stream.
 FETCH_SIZE / WRITE_SIZE counters (each scaled by the factor the same counter
 reads for a plain copy of known size), read from the committed counter summary
 of the same workload when it is present.
-``cpu_baseline`` times the FP64 oracle's components on a bounded sample of the
-same workload on the host cores and projects them to the same iteration count.
+``cpu_baseline`` times the FP64 oracle with the reference's arithmetic on the
+host cores: the setup, one H*X of the whole block, one Rayleigh-Ritz pass, the
+Hartree CG, the XC point loop and a mixer update of the same workload, each in
+full, summed to the FP64 run's iteration count; and the H2O problem run to
+convergence in full.
 
 The Hartree solve runs to the relative tolerance 1e-5 with an iteration cap of
 1000 in every configuration.  The plain FP64 solve uses the unpreconditioned
@@ -86,79 +93,174 @@ def measured_traffic(name):
         return json.load(f).get("traffic_bytes_per_launch")
 
 
-def solve(prob, timer=None, net=None):
+def solve(prob, timer=None, net=None, **kw):
     """One timed solve; with ``net`` the learned start (network forward pass and
-    its Hartree correction) is inside the timed region."""
+    its Hartree correction) is inside the timed region.  ``kw`` goes to run_scf."""
     import torch
     from parsec_python_amd.scf import run_scf
     prob.H.stencil_timer = timer
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     if net is None:
-        res = run_scf(prob)
+        res = run_scf(prob, **kw)
     else:
         from parsec_python_amd.learned import learned_start
         rho1, vh1 = learned_start(prob, net)
-        res = run_scf(prob, rho_init=rho1, hpot_init=vh1)
+        res = run_scf(prob, rho_init=rho1, hpot_init=vh1, **kw)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prob.H.stencil_timer = None
     return res, t1 - t0
 
 
-def cpu_baseline(name, iterations, cg_its, first_filt_outer):
-    """Time the FP64 oracle's per-iteration components on the host (bounded
-    sample) and project to ``iterations`` SCF iterations of the workload."""
+def _host_info():
+    """CPU model, threads and library versions of the host the CPU leg ran on."""
+    import platform
+
+    import numpy as np
+    import scipy
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "threads_available": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS"),
+            "OPENBLAS_NUM_THREADS": os.environ.get("OPENBLAS_NUM_THREADS"),
+            "numpy": np.__version__, "scipy": scipy.__version__}
+
+
+def cpu_baseline(name, iterations, first_filt_outer, h2o=True):
+    """The FP64 oracle with the reference's arithmetic, timed on the host cores.
+
+    Measured in full on the workload (nothing projected from a smaller
+    operator): the setup (SAD / V_loc / V_H0 spline evaluation and the KB
+    projector matrix), one H*X of the whole block with the reference's CSR
+    operator 0.5 A + V_nl + diag(pot) (V_nl as the dense per-atom outer
+    products of src/V_ion/pseudoNL_original.py:192-206, SciPy SpMM), one
+    Rayleigh-Ritz pass as the reference does it (scipy QR, the k(k+1)/2 vdot
+    loop of src/Eigensolvers/chefsi1.py:51-53, eigh, W Q), the Hartree CG to
+    1e-5 (plain CG, src/Eigensolvers/pcg.py), the XC point loop
+    (src/V_xc/exc_nspn.py:73-91), the density and one mixer update.  A chefsi1
+    SCF iteration is 10 filter steps + 6 Lanczos matvecs + the RR pass (its
+    own H*X included), first_filt is ``first_filt_outer`` passes of 10 filter
+    steps + the RR pass; the time to converge adds these for the FP64 GPU
+    run's iteration count.  With ``h2o`` the BASELINE H2O problem (sphere,
+    69,264 points, nev 8) is also run to |dE| < 1e-6 Ha/atom in full."""
     import numpy as np
     import scipy.linalg as sl
     import scipy.sparse as sp
-    from oracle import fd as ofd, grid as ogrid, pseudo as opseudo, xc as oxc
+    from oracle import eig as oeig, fd as ofd, grid as ogrid, pseudo as opseudo, scf as oscf, xc as oxc
+    from oracle.mixer import Msecant1
+    from oracle.pcg import pcg as opcg
     atoms, dom, nev = _atoms(name)
     k = nev + 15
+    t = {}
+    t0 = time.perf_counter()
     mask = ogrid.mask_flat(dom, "sphere")
     n = int(mask.sum())
-    A = (1.0 / dom["h"] ** 2) * ofd.laplacian_masked(dom["nx"], dom["ny"], dom["nz"], 4, mask)
+    A = (1.0 / dom["h"] ** 2) * ofd.laplacian_masked(dom["nx"], dom["ny"], dom["nz"], 4, mask).tocsr()
+    rho0, hpot0, ppot, _ = opseudo.pseudo_diag(dom, atoms, mask)
+    vnl = opseudo.pseudo_nl(dom, atoms, mask)
+    half = (0.5 * A + vnl).tocsr()
+    t["setup"] = time.perf_counter() - t0
     rng = np.random.default_rng(0)
-    pot = rng.standard_normal(n)
-    H = (0.5 * A + sp.diags(pot)).tocsr()
-    tabs = opseudo.projector_tables(dom, atoms, mask)
-    kb = 16
-    X = rng.standard_normal((n, kb))
-
-    def tmin(f, reps=2):
-        best = 1e30
-        for _ in range(reps):
-            t = time.perf_counter()
-            f()
-            best = min(best, time.perf_counter() - t)
-        return best
-
-    t_hx = tmin(lambda: H @ X + opseudo.apply_nl(tabs, X)) / kb          # per vector H*x
-    kq = min(k, 48)
-    Q = rng.standard_normal((n, kq))
-    t_qr = tmin(lambda: sl.qr(Q, mode="economic"), 1) * (k / kq) ** 2     # QR cost ~ n k^2
-    t_gemm = tmin(lambda: Q.T @ Q, 1) * (k / kq) ** 2                    # one n x k x k GEMM
+    pot = ppot + hpot0
+    B = (half + sp.diags(pot, 0, shape=(n, n))).tocsr()
+    X = rng.standard_normal((n, k))
+    t0 = time.perf_counter()
+    HX = B @ X
+    t["hx_block"] = time.perf_counter() - t0
+    del HX
     v = rng.standard_normal(n)
-    t_cg = tmin(lambda: (A @ v, v @ v, v @ v, v + 0.1 * v, v - 0.1 * v, v + 0.1 * v), 3)
-    rho = np.abs(v) * 1e-2
-    t_xc = tmin(lambda: oxc.exc_nspn(dom["h"], rho), 1)
-    # per chefsi1 iteration: 10 filter steps + 1 RR apply + 6 Lanczos matvecs, QR, G and rotation GEMMs
-    t_diag = (10 * k + k + 6) * t_hx + t_qr + 2 * t_gemm
-    t_iter = t_diag + cg_its * t_cg + t_xc
-    # iteration 1 (first_filt): 6 Lanczos matvecs + passes of (10 filter steps + 1 RR apply, QR, 2 GEMMs)
-    t_first = 6 * t_hx + first_filt_outer * ((10 * k + k) * t_hx + t_qr + 2 * t_gemm)
-    total = t_first + cg_its * t_cg + t_xc + (iterations - 1) * t_iter
-    threads = len(os.sched_getaffinity(0))
-    return {"value": round(total, 3), "unit": "s", "cores": threads, "kind": "port",
-            "sample": (f"FP64 oracle (NumPy/SciPy, {threads} host threads for BLAS) component timings on the {name} "
-                       f"operator: H*X with {kb} vectors (CSR Laplacian+diag, factored projectors), QR and Gram on "
-                       f"n x {kq} scaled by (k/{kq})^2, one CG iteration, one XC pass; projected to {iterations} SCF "
-                       f"iterations with {cg_its} CG iterations each and {first_filt_outer} first_filt passes")}
+    t0 = time.perf_counter()
+    for _ in range(6):
+        B @ v
+    t["lanczos_6"] = time.perf_counter() - t0
+    # Rayleigh-Ritz as chefsi1.py:40-69: QR, H W, the vdot double loop, eigh, rotation
+    t0 = time.perf_counter()
+    W, _ = sl.qr(X, mode="economic")
+    V = B @ W
+    G = np.zeros((k, k))
+    for i in range(k):
+        for j in range(i, k):
+            G[i, j] = np.vdot(V[:, i], W[:, j])
+            G[j, i] = G[i, j]
+    lam, Q = sl.eigh(G)
+    W = W @ Q
+    t["rr_pass"] = time.perf_counter() - t0
+    occ = np.zeros(nev)
+    occ[: int(round(opseudo.nelectrons(atoms) / 2))] = 1.0
+    t0 = time.perf_counter()
+    rho = (W[:, :nev] ** 2) @ (2 * occ)
+    t["density"] = time.perf_counter() - t0
+    del V, W, X
+    h = dom["h"]
+    hrhs = (4 * np.pi / h ** 3) * (rho0 * 1.001 - rho0)
+    t0 = time.perf_counter()
+    _, cg_its = opcg(A, hrhs, np.zeros(n), PCG_MAXIT, 1e-5, reference_aliasing=False)
+    t["hartree_cg"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vxc, _ = oxc.exc_nspn_loop(h, rho0 / h ** 3)
+    t["xc_loop"] = time.perf_counter() - t0
+    mx = Msecant1()
+    for _ in range(3):
+        mx(pot, 1e-3 * rng.standard_normal(n))
+    t0 = time.perf_counter()
+    mx(pot, 1e-3 * rng.standard_normal(n))
+    t["mixer"] = time.perf_counter() - t0
+    filt10 = 10 * t["hx_block"]
+    per_iter_rest = t["density"] + t["hartree_cg"] + t["xc_loop"] + t["mixer"]
+    t_chefsi1 = filt10 + t["lanczos_6"] + t["rr_pass"]
+    t_first = first_filt_outer * (filt10 + t["rr_pass"])
+    total = t["setup"] + t_first + (iterations - 1) * t_chefsi1 + iterations * per_iter_rest
+    info = _host_info()
+    out = {"value": round(total, 1), "unit": "s", "cores": info["threads_available"], "kind": "port",
+           "host": info,
+           "measured_s": {kk: round(vv, 3) for kk, vv in t.items()},
+           "one_scf_iteration_s": round(t_chefsi1 + per_iter_rest, 2),
+           "sample": (f"FP64 oracle with the reference's arithmetic on the {name} workload ({n} points, k {k}): setup, "
+                      f"one H*X of all {k} vectors (SciPy CSR SpMM, single-threaded as in the reference), one "
+                      f"Rayleigh-Ritz pass (QR, vdot loop, eigh, rotation; BLAS on the host threads), the Hartree CG "
+                      f"({cg_its} iterations), the XC point loop and a mixer update, each timed in full; time to "
+                      f"converge = setup + {first_filt_outer} first_filt passes + {iterations - 1} chefsi1 "
+                      f"iterations (10 filter steps each) + {iterations} x (density, CG, XC, mixer)")}
+    if h2o:
+        a2, d2, nev2 = _atoms("h2o")
+        t0 = time.perf_counter()
+        st = oscf.build_setup(a2, d2, "sphere", 8)
+        r2 = oscf.run_scf(st, nev2, tol=1e-12, maxits=60, poldeg=10, seed=1, etol_ha_per_atom=1e-6,
+                          pcg_aliasing=True, xc_loop=True)
+        out["h2o_time_to_converge_s"] = round(time.perf_counter() - t0, 2)
+        out["h2o"] = {"points": int(st.A.shape[0]), "iterations": r2.iterations, "converged": bool(r2.converged),
+                      "setup_and_scf_s": out["h2o_time_to_converge_s"],
+                      "workload": WORKLOADS["h2o"]["desc"] + ", FP64 oracle, stop |dE| < 1e-6 Ha/atom"}
+    return out
 
 
 def _mean_cg(res) -> float:
     its = [h["cg_its"] for h in res.history if "cg_its" in h]
     return round(float(sum(its)) / len(its), 1) if its else 0.0
+
+
+def _repeat(prob, reps, warm=1, label="", **kw):
+    """``warm`` untimed-for-the-mean solves, then ``reps`` timed ones; returns
+    (last result, mean time, first (cold) time)."""
+    cold = None
+    for i in range(warm):
+        r, t = solve(prob, **kw)
+        cold = t if cold is None else cold
+        _log(f"{label} warm-up: {r.iterations} iterations, {t:.3f} s")
+    times, res = [], None
+    for _ in range(reps):
+        res, t = solve(prob, **kw)
+        times.append(t)
+        _log(f"{label}: {res.iterations} iterations, {t:.3f} s")
+    return res, float(sum(times) / len(times)), cold
 
 
 def main():
@@ -183,12 +285,13 @@ def main():
 
     atoms, dom, nev = _atoms(args.workload)
     etol = 1e-6
+    # the baselines and controls: one warm-up solve, then the mean of `reps` solves
+    reps = max(1, min(args.steps, 5))
     common = dict(tol=1e-12, maxits=60, domain="sphere", seed=1, etol_ha_per_atom=etol, pcg_maxit=PCG_MAXIT,
                   pcg_aliasing=False)
     prob64 = SCFProblem(atoms, dom, nev, SCFConfig(precision="fp64", **common))
     _log(f"setup {prob64.setup_time:.2f} s ({prob64.n} points, k {prob64.k})")
-    res64, t64 = solve(prob64)
-    _log(f"fp64 solve: {res64.iterations} iterations, {t64:.3f} s")
+    res64, t64, t64_cold = _repeat(prob64, reps, label="fp64")
     prob64.close()
     del prob64
     torch.cuda.empty_cache()
@@ -215,15 +318,17 @@ def main():
     # the same FP64 solve with the multigrid-preconditioned Hartree CG: separates
     # the preconditioner's share of the speed-up from that of the precision and start
     prob64m = SCFProblem(atoms, dom, nev, SCFConfig(precision="fp64", **dict(common, pcg_prec="mg")))
-    res64m, t64m = solve(prob64m)
-    _log(f"fp64 + multigrid CG: {res64m.iterations} iterations, {t64m:.3f} s")
+    res64m, t64m, _ = _repeat(prob64m, reps, label="fp64 + multigrid CG")
     prob64m.close()
     del prob64m
     torch.cuda.empty_cache()
 
     prob = SCFProblem(atoms, dom, nev, SCFConfig(precision="mixed", **dict(common, pcg_prec="mg")))
-    res_sad, t_sad = solve(prob)
-    _log(f"mixed from SAD: {res_sad.iterations} iterations, {t_sad:.3f} s")
+    res_sad, t_sad, _ = _repeat(prob, reps, label="mixed from SAD")
+    # control: SAD start with the start vectors weighted by sqrt(rho_SAD), the
+    # weighting the learned start uses, so the network's own share is visible
+    res_sadw, t_sadw, _ = _repeat(prob, reps, label="mixed from SAD, SAD-weighted start vectors",
+                                  start_weight_rho=prob.rho0)
     for _ in range(args.warmup):
         r, t = solve(prob, net=net)
         _log(f"mixed warmup: {r.iterations} iterations, {t:.3f} s")
@@ -247,6 +352,8 @@ def main():
     occ = res64.occupations > 0.5
     dlam = float(np.max(np.abs(res.eigenvalues[:nev][occ] - res64.eigenvalues[:nev][occ]))) if occ.any() else 0.0
     value = float(np.mean(times))
+    start = ("learned start (the network's forward pass and its Hartree correction inside the timed region)"
+             if net is not None else "SAD start")
     out = {
         "metric": METRIC,
         "value": round(value, 4),
@@ -259,20 +366,28 @@ def main():
         "scaling": "strong",
         "vs_baseline": None,
         "dtype": "fp32+fp64" + ("+bf16" if net is not None else ""),
-        "data": "synthetic geometry (built-in coordinates), SAD start",
+        "data": f"synthetic geometry (built-in coordinates), {start}",
         "config": {"workload": WORKLOADS[args.workload]["desc"], "domain": "sphere", "points": prob.n,
                    "block_k": nev + 15, "learned_init": net is not None,
                    "learned_checkpoint": os.path.basename(ckpt) if net is not None else None,
                    "stop": "|dE| < 1e-6 Ha/atom between SCF iterations"},
         "fp64_time_to_converge_s": round(t64, 4),
+        "fp64_first_solve_s": round(t64_cold, 4),
         "speedup_vs_fp64": round(t64 / value, 3),
+        "speedup_vs_fp64_multigrid": round(t64m / value, 3),
+        "baselines_timed_as": f"one warm-up solve, then the mean of {reps} solves (value: {args.warmup} warm-up, "
+                              f"mean of {args.steps})",
         "hartree_cg": {"fp64": "plain CG (reference cg_prec=0)", "mixed": "multigrid-preconditioned CG",
                        "fp64_multigrid_time_s": round(t64m, 4),
                        "cg_iterations_mean": {"fp64": _mean_cg(res64), "fp64_multigrid": _mean_cg(res64m),
                                               "mixed": _mean_cg(res)}},
         "iterations": {"mixed": res.iterations, "fp64": res64.iterations, "fp64_multigrid": res64m.iterations,
-                       "mixed_sad_start": res_sad.iterations},
-        "mixed_sad_start_time_s": round(t_sad, 4),
+                       "mixed_sad_start": res_sad.iterations, "mixed_sad_weighted_start": res_sadw.iterations},
+        "start_controls_s": {"mixed_sad": round(t_sad, 4), "mixed_sad_weighted": round(t_sadw, 4),
+                             "mixed_learned": round(value, 4)},
+        "learned_share": {"iterations_saved_vs_sad_weighted": res_sadw.iterations - res.iterations,
+                          "speedup_vs_sad_weighted": round(t_sadw / value, 3),
+                          "weighting_share_speedup_sad_to_sad_weighted": round(t_sad / t_sadw, 3)},
         "ms_per_iteration": {"mixed": round(1e3 * value / res.iterations, 3),
                              "fp64": round(1e3 * t64 / res64.iterations, 3)},
         "error_vs_fp64": {"energy_ha_per_atom": abs(e_mixed - e_fp64), "max_occupied_eig_ha": dlam,
@@ -285,12 +400,12 @@ def main():
                      "launches": int(len(durs)), "avg_launch_us": round(1e3 * float(durs.mean()), 2) if len(durs) else None,
                      "algorithmic_bytes_per_launch": avg_bytes},
     }
+    prob.close()
     if not args.no_cpu_baseline:
         first = int(res64.setup_info.get("first_filt", {}).get("outer_iterations", 5))
-        cg = int(round(np.mean([h["cg_its"] for h in res64.history])))
-        _log("timing the CPU baseline sample")
-        out["cpu_baseline"] = cpu_baseline(args.workload, res64.iterations, cg, first)
-    prob.close()
+        _log("timing the CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(args.workload, res64.iterations, first)
+        out["cpu_baseline"]["vs_gpu_fp64"] = round(out["cpu_baseline"]["value"] / t64, 1)
     print(json.dumps(out))
 
 

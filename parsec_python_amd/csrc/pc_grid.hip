@@ -24,6 +24,7 @@
 #include "pc_common.h"
 #include "pc_zmarch.h"
 #include <type_traits>
+#include <utility>
 #include <vector>
 #include <algorithm>
 
@@ -44,6 +45,7 @@ struct GridDev {
 
 struct StArgs {
   int nx, ny, nz;
+  long long npts;
   const int* lo;
   const int* hi;
   const int* off;
@@ -333,6 +335,278 @@ __global__ __launch_bounds__(64 * NW) void stencil_zmarch(StArgs g, const T* __r
   }
 }
 
+
+// ---- the lean z-march (one vector per workgroup, no partial sums) ------------
+// The same computation as stencil_zmarch<T,R,SX,1,1,2,HASX,false,1,NW>, written
+// to spend fewer vector instructions per plane:
+//   * the z-queue is a ring of QL registers per x-chunk, indexed by the plane's
+//     phase; the plane loop is unrolled QL times so the ring turns by renaming
+//     instead of register moves (slot (z - zb + R) mod QL holds plane z);
+//   * every global access is a raw buffer access on a per-vector resource whose
+//     range is the vector's npts points: a lane outside its row's chord gets an
+//     out-of-range offset, so its load returns 0 and its store is dropped, with
+//     no branches around the memory instructions;
+//   * X and V are read at unmasked offsets (their values only reach stored
+//     lanes); X, V and the halo rows of plane z+2 go out before plane z is
+//     computed, and the queue front of plane z-R+QL right after plane z's last
+//     use of slot z-R (QL-2R-1 planes ahead of its use).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pc_rsrc(const void* p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T pc_bload(__amdgpu_buffer_rsrc_t r, int off) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+}
+template <typename T>
+__device__ __forceinline__ void pc_bstore(__amdgpu_buffer_rsrc_t r, int off, T v) {
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, off, 0, 0);
+  } else {
+    using V2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(V2, v), r, off, 0, 0);
+  }
+}
+
+template <int... I, class F>
+__device__ __forceinline__ void pc_unroll_seq(std::integer_sequence<int, I...>, F&& f) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+
+#ifndef PC_LEAN_NS
+#define PC_LEAN_NS 3
+#endif
+#ifndef PC_LEAN_PF
+#define PC_LEAN_PF 3
+#endif
+#ifndef PC_LEAN_WAVES
+#define PC_LEAN_WAVES 0
+#endif
+// queue length: 2R+1 planes plus PC_LEAN_PF fronts in flight, rounded up to a
+// multiple of 2*NS (tile parity and register sets repeat within the block)
+constexpr int pc_lean_ql(int R) {
+  return (2 * R + 1 + PC_LEAN_PF + 2 * PC_LEAN_NS - 1) / (2 * PC_LEAN_NS) * (2 * PC_LEAN_NS);
+}
+
+// Workgroup: TY y-rows of one vector; wave w owns row w % TY and the CPW
+// x-chunks of 64 points starting at chunk (w / TY) * CPW, so a workgroup has
+// TY * SX / CPW waves and each wave keeps only CPW chunks of queue registers.
+template <typename T, int R, int SX, int TY, int CPW, bool HASX>
+__global__ __launch_bounds__(64 * TY * SX / CPW)
+#if PC_LEAN_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(PC_LEAN_WAVES, PC_LEAN_WAVES)))
+#endif
+void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T cb, T cc) {
+  constexpr int NWV = TY * SX / CPW;          // waves per workgroup
+  constexpr int NXP = 64 * SX;
+  constexpr int LW = NXP + 2 * R;
+  constexpr int LR = TY + 2 * R;
+  constexpr int NHI = 2 * R * (SX / CPW);     // halo items: (halo row, chunk group)
+  constexpr int HT = (NHI + NWV - 1) / NWV;   // halo items per wave
+  constexpr int NS = PC_LEAN_NS;
+  constexpr int QL = pc_lean_ql(R);
+  constexpr int ES = (int)sizeof(T);
+  constexpr int OOB = 0x7ffffff0;     // beyond every range (npts * ES < 2^31 is checked on the host)
+  static_assert(SX % CPW == 0, "chunks per wave must divide the chunks per row");
+  __shared__ T tile[2][LR * LW];
+  // row metadata of the slab's LR rows for the planes the march touches (the
+  // launch sizes it for the grid's nz), read with uniform addresses every plane
+  extern __shared__ int2 mtab[];
+
+  const int bid = blockIdx.x;
+  const int L = (bid & 7) * g.nb8 + (bid >> 3);
+  if (L >= g.nblocks) return;
+  const int v = L / g.nslab;
+  const int slab = L - v * g.nslab;
+  const int j0 = slab * TY;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ny = g.ny, nz = g.nz;
+
+  int zb = nz, ze = 0;
+  for (int t = 0; t < TY; ++t) {
+    const int j = j0 + t;
+    if (j < ny) {
+      zb = min(zb, g.zrange[2 * j]);
+      ze = max(ze, g.zrange[2 * j + 1]);
+    }
+  }
+  if (zb >= ze) return;
+  const int zlo = zb - R;
+  // planes zb-R .. zb-R + QL*(ceil((ze-zb)/QL) + 1) - 1: the launch sizes the
+  // table for nz + 2 QL planes
+  const int zm = (ze - zb + QL - 1) / QL * QL + QL;
+
+  const unsigned nbytes = (unsigned)(g.npts * ES);
+  const long long base = (long long)v * g.ld;
+  const __amdgpu_buffer_rsrc_t ry = pc_rsrc(Y + base, nbytes);
+  const __amdgpu_buffer_rsrc_t rx = pc_rsrc(HASX ? X + base : Y + base, nbytes);
+  const __amdgpu_buffer_rsrc_t ro = pc_rsrc(Out + base, nbytes);
+  const __amdgpu_buffer_rsrc_t rv = pc_rsrc(V, nbytes);
+
+  for (int t = threadIdx.x; t < 2 * LR * LW; t += 64 * NWV) (&tile[0][0])[t] = T(0);
+  for (int e = threadIdx.x; e < LR * zm; e += 64 * NWV) {
+    const int t = e / zm, z = zlo + (e - t * zm);
+    const int j = j0 - R + t;
+    mtab[e] = (j >= 0 && j < ny && z >= 0 && z < nz) ? g.meta[j + ny * z] : make_int2(0, 0);
+  }
+  __syncthreads();
+
+  // metadata of slab row t (0 .. LR-1, own rows at R .. R+TY-1) at plane z, as scalars
+  auto mrow = [&](int t, int z) -> int2 {
+    const int2 m = mtab[t * zm + (z - zlo)];
+    return make_int2(__builtin_amdgcn_readfirstlane(m.x), __builtin_amdgcn_readfirstlane(m.y));
+  };
+  // byte offset of x-chunk c in a row with metadata m: masked (out of range
+  // outside the chord) or plain (any in-range or out-of-range address)
+  auto off_masked = [&](int2 m, int c) -> int {
+    const int i = lane + 64 * c;
+    const int lo = m.y & 0xffff, len = (m.y >> 16) - lo;
+    return (unsigned)(i - lo) < (unsigned)len ? (m.x + i) * ES : OOB;
+  };
+  auto off_plain = [&](int2 m, int c) -> int { return (m.x + lane + 64 * c) * ES; };
+
+  const int town = R + wv % TY;               // own row in the tile
+  const int c0 = (wv / TY) * CPW;              // first own chunk
+  int th[HT], tc[HT];                          // halo items: tile row and first chunk
+#pragma unroll
+  for (int u = 0; u < HT; ++u) {
+    const int k = wv + NWV * u;
+    const int hr = k % (2 * R);
+    th[u] = hr < R ? hr : hr + TY;
+    tc[u] = (k / (2 * R)) * CPW;
+  }
+  auto has_halo = [&](int u) { return wv + NWV * u < NHI; };
+
+  T q[CPW][QL];
+  T xq[NS][CPW], vq[NS][CPW], hq[NS][HT][CPW];
+  int so[NS][CPW];
+  // prefill: planes zb-R .. zb-R+QL-1 into slots 0 .. QL-1 (slot of plane p: (p - zb + R) mod QL)
+#pragma unroll
+  for (int d = 0; d < QL; ++d) {
+    const int2 m = mrow(town, zb - R + d);
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) q[c][d] = pc_bload<T>(ry, off_masked(m, c0 + c));
+  }
+  // X, V, halo rows and store offsets of plane z into set S
+  auto issue_plane = [&](auto set, int z) {
+    constexpr int S = decltype(set)::value;
+    const int2 mo = mrow(town, z);
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const int po = off_plain(mo, c0 + c);
+      so[S][c] = off_masked(mo, c0 + c);
+      vq[S][c] = pc_bload<T>(rv, po);
+      if constexpr (HASX) xq[S][c] = pc_bload<T>(rx, po);
+    }
+#pragma unroll
+    for (int u = 0; u < HT; ++u) {
+      if (has_halo(u)) {
+        const int2 mh = mrow(th[u], z);
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) hq[S][u][c] = pc_bload<T>(ry, off_masked(mh, tc[u] + c));
+      }
+    }
+  };
+  pc_unroll_seq(std::make_integer_sequence<int, NS - 1>{}, [&](auto k) {
+    constexpr int K = decltype(k)::value;
+    issue_plane(std::integral_constant<int, K>{}, zb + K);
+  });
+
+  const T w0 = T(g.w[0]);
+  T wd[R + 1];
+#pragma unroll
+  for (int d = 1; d <= R; ++d) wd[d] = T(g.w[d]);
+
+  // Planes run in whole blocks of QL, past ze up to the next multiple of QL:
+  // there the slab's rows are empty, so those planes' stores are all out of
+  // range, and the loads they issue read zeros or stay inside the vectors.
+  // No branch separates a load from its use, which keeps the compiler from
+  // sinking the prefetches to their uses.
+  auto plane = [&](auto phase, int z) {
+    constexpr int P = decltype(phase)::value;
+    constexpr int S = P % NS;      // (z - zb) mod NS: the register set of plane z
+    constexpr int B = P & 1;       // LDS tile of this plane
+    issue_plane(std::integral_constant<int, (P + NS - 1) % NS>{}, z + NS - 1);
+    __builtin_amdgcn_sched_barrier(0);
+    T* tb = tile[B];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) tb[town * LW + R + lane + 64 * (c0 + c)] = q[c][(P + R) % QL];
+#pragma unroll
+    for (int u = 0; u < HT; ++u) {
+      if (has_halo(u)) {
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) tb[th[u] * LW + R + lane + 64 * (tc[u] + c)] = hq[S][u][c];
+      }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+      const T* trow = tb + town * LW + R + lane + 64 * (c0 + c);
+      const T yc = q[c][(P + R) % QL];
+      T acc = w0 * yc;
+#pragma unroll
+      for (int d = 1; d <= R; ++d) {
+        const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[c][(P + R - d) % QL] +
+                     q[c][(P + R + d) % QL];
+        acc += wd[d] * nb;
+      }
+      T o = acc + (ca * vq[S][c] + cb) * yc;
+      if constexpr (HASX) o += cc * xq[S][c];
+      pc_bstore<T>(ro, so[S][c], o);
+    }
+    // slot P (plane z-R) is free: plane z-R+QL, the queue front of plane z-2R+QL
+    {
+      const int2 m = mrow(town, z - R + QL);
+#pragma unroll
+      for (int c = 0; c < CPW; ++c) q[c][P] = pc_bload<T>(ry, off_masked(m, c0 + c));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  for (int z0 = zb; z0 < ze; z0 += QL) {
+    pc_unroll_seq(std::make_integer_sequence<int, QL>{}, [&](auto ph) {
+      constexpr int P = decltype(ph)::value;
+      plane(ph, z0 + P);
+    });
+  }
+}
+
+#ifndef PC_LEAN_TY
+#define PC_LEAN_TY 8
+#endif
+
+template <typename T, int R, int SX>
+int launch_lean(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+                double ca, double cb, double cc, hipStream_t st) {
+  constexpr int TY = PC_LEAN_TY;
+  constexpr int CPW = SX > 2 ? SX / 2 : 1;     // at most 16 waves (1024 threads) per workgroup
+  constexpr int QL = pc_lean_ql(R);
+  StArgs a;
+  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
+  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
+  a.ld = ld;
+  a.nslab = (gd->ny + TY - 1) / TY;
+  a.nvec = nvec;
+  a.nblocks = a.nslab * nvec;
+  a.nb8 = (a.nblocks + 7) / 8;
+  for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
+  a.w[0] = 3.0 * gd->w[0] * scale;
+  for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
+  const dim3 grid(8 * a.nb8), block(64 * TY * SX / CPW);
+  const size_t mbytes = sizeof(int2) * (size_t)(TY + 2 * R) * (size_t)(gd->nz + 2 * QL);
+  if (X)
+    hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, true>), grid, block, mbytes, st, a, Y, X, Out, V, T(ca),
+                       T(cb), T(cc));
+  else
+    hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, false>), grid, block, mbytes, st, a, Y, X, Out, V, T(ca),
+                       T(cb), T(cc));
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
 template <typename T, int R, int SX, int TYW, int NV, int NB = 2, int NW = 4>
 constexpr int zmarch_lds_bytes() {
   return NB * NV * (NW * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
@@ -343,7 +617,7 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
                double ca, double cb, double cc, double* dot_part, hipStream_t st) {
   constexpr int TY = NW * TYW;
   StArgs a;
-  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
+  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
   a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
   a.ld = ld;
   a.nslab = (gd->ny + TY - 1) / TY;
@@ -401,6 +675,9 @@ int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
   constexpr int NW = PC_STENCIL_NW;
   if (dot_part)   // per-vector partial sums need one vector per workgroup
     return launch_cfg<T, R, SX, 1, 1, 2, 1, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+#ifndef PC_STENCIL_NO_LEAN
+  if constexpr (SX <= 2) return launch_lean<T, R, SX>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, st);
+#endif
   if constexpr (PC_STENCIL_NV >= 2 && zmarch_lds_bytes<T, R, SX, 1, 2, 2, NW>() <= kLds)
     return launch_cfg<T, R, SX, 1, 2, 2, 1, NW>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
   else if constexpr (zmarch_lds_bytes<T, R, SX, 1, 1, 2, NW>() <= kLds)
@@ -426,6 +703,7 @@ int stencil_dispatch(void* grid, const T* Y, const T* X, T* Out, const T* V, int
   PC_CHECK_ARG(Y != Out, "output must not alias the stencil input");
   PC_CHECK_ARG(nvec >= 1, "nvec < 1");
   PC_CHECK_ARG(ld >= gd->npts, "ld < npts");
+  PC_CHECK_ARG(gd->npts * (long long)sizeof(T) < 0x7ffffff0LL, "vector larger than 2 GiB");
   hipStream_t st = static_cast<hipStream_t>(stream);
   switch (gd->radius) {
     case 1: return launch_r<T, 1>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
@@ -498,7 +776,7 @@ template <typename T, int R>
 int launch_direct_r(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld,
                     double scale, double ca, double cb, double cc, double* dot_part, hipStream_t st) {
   StArgs a;
-  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz;
+  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
   a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
   a.ld = ld; a.nslab = 0; a.nvec = nvec; a.nblocks = 0; a.nb8 = 0;
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
