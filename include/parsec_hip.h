@@ -105,6 +105,11 @@ int pc_dense_set_method(void* ctx, int method);
  * workgroup; reset = 1 re-enables it; out_aborts = reductions repeated so far.
  * No counterpart in the reference. */
 int pc_dense_trd_control(void* ctx, int spins, int reset, int* out_aborts);
+/* pc_dense_rr repeats a step whose overlap matrix is not numerically positive
+ * definite with S + shift*max(diag S)*I, shift 1e-13, 1e-11, 1e-9 (the
+ * reference's Householder QR in src/Eigensolvers/chefsi1.py:43 does not fail on
+ * nearly dependent columns); out = steps of this context that needed it. */
+int pc_dense_shift_count(void* ctx, int* out);
 
 /* ---- point-wise SCF kernels --------------------------------------------------
  * density: rho = sum_v occ2[v] W_v^2           (src/rsdft_solver.py:324)

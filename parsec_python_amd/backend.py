@@ -25,8 +25,9 @@ reference's SCF driver (``src/rsdft_solver.py:281-395``) can run unchanged:
 * ``pcg(A, rhs, x0, maxit, tol, *prec)`` -> ``(x, its)`` on the device; the
   preconditioner arguments are ignored with a one-time warning, as the
   reference's GPU twin does (``src/Eigensolvers/pcg_gpu.py:40-44``).
-* ``lanczos`` and ``chsubsp`` raise ``NotImplementedError`` (diagmeth 0/1/2 are
-  not part of this backend; diagmeth 3, the default, is).
+* ``lanczos(B, nev, v, m, tol)`` and ``chsubsp(deg, nev, B)`` (diagmeth 0/1/2)
+  run the device Lanczos of ``Eigensolver.lanczos`` / ``Eigensolver.chsubsp``
+  and return ``(W, ritz)`` like ``first_filt``.
 
 ``to_numpy_array`` / ``to_host_scalar`` accept torch tensors.  Failures of the
 HIP library raise ``ParsecHipError``; there is no CPU fallback.
@@ -330,11 +331,22 @@ class GpuBackendImpl:
             return v.detach().cpu().numpy()
         return v
 
-    def lanczos(self, *args, **kwargs):
-        raise NotImplementedError("lanczos (diagmeth 0/1) is not provided by the HIP backend; use diagmeth 3")
+    # -- src/Eigensolvers/lanczos.py:9-223 (diagmeth 0 at the first iteration, 1 at every one)
+    def lanczos(self, B, nev, v, m, tol, *progress):
+        """nev Ritz pairs from m Lanczos steps; ``v`` (the reference's random
+        start vector) is not used: the start vector comes from the backend's
+        seeded counter-based generator."""
+        H = self._operator(B)
+        es = self.ctx.eigensolver(nev)
+        W, ritz = es.lanczos(int(nev), int(m), float(tol))
+        return self._w_view(W), np.asarray(ritz, dtype=np.float64)
 
-    def chsubsp(self, *args, **kwargs):
-        raise NotImplementedError("chsubsp (diagmeth 2) is not provided by the HIP backend; use diagmeth 3")
+    # -- src/Eigensolvers/chsubsp.py:10-91 (diagmeth 2, first iteration)
+    def chsubsp(self, deg, nev, B):
+        H = self._operator(B)
+        es = self.ctx.eigensolver(nev)
+        W, ritz = es.chsubsp(int(nev), int(deg))
+        return self._w_view(W), np.asarray(ritz, dtype=np.float64)
 
 
 def make_backend(precision: str = "fp64", seed: int = 0, pcg_aliasing: bool = True,

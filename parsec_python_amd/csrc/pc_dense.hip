@@ -34,7 +34,25 @@ struct DenseCtx {
   bool mwg_ok;           // false after a multi-workgroup reduction gave up waiting
   int mwg_aborts;        // reductions repeated with the one-workgroup kernel
   int trd_spin;          // polls of one wait before sytrd_lower_mwg gives up
+  double* S0;            // kmax*kmax: copy of the overlap matrix (shifted retry)
+  double* G0;            // kmax*kmax: copy of the projected matrix (shifted retry)
+  int shifted;           // Rayleigh-Ritz steps that needed a shifted Cholesky factor
 };
+
+// S += shift * max_i S_ii on the diagonal (one workgroup of 256 threads)
+__global__ void add_diag_shift(double* S, int k, int lds, double shift) {
+  __shared__ double red[256];
+  double m = 0.0;
+  for (int i = threadIdx.x; i < k; i += 256) m = fmax(m, S[(long long)i * lds + i]);
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const double d = shift * red[0];
+  for (int i = threadIdx.x; i < k; i += 256) S[(long long)i * lds + i] += d;
+}
 
 __global__ void symmetrize(double* G, int k, int ldg) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -522,6 +540,9 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipHostMalloc(&c->host_out, (size_t)(kmax + 4) * sizeof(double), hipHostMallocDefault));
   PC_HIP(hipMalloc(&c->sync, kTrdSyncInts * sizeof(int)));
   PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
+  PC_HIP(hipMalloc(&c->S0, (size_t)kmax * kmax * sizeof(double)));
+  PC_HIP(hipMalloc(&c->G0, (size_t)kmax * kmax * sizeof(double)));
+  c->shifted = 0;
   c->mwg_ok = true;
   c->mwg_aborts = 0;
   c->trd_spin = kTrdSpin;
@@ -538,6 +559,7 @@ int pc_dense_destroy(void* ctx) {
   rocblas_destroy_handle(c->h);
   (void)hipFree(c->tmp); (void)hipFree(c->E); (void)hipFree(c->out); (void)hipFree(c->info);
   (void)hipFree(c->resid); (void)hipFree(c->sweeps); (void)hipFree(c->tau); (void)hipFree(c->P);
+  (void)hipFree(c->S0); (void)hipFree(c->G0);
   (void)hipHostFree(c->host_out);
   delete c;
   return pc::PC_OK;
@@ -546,6 +568,13 @@ int pc_dense_destroy(void* ctx) {
 // S (k x k, ld lds) and Gy (k x k, ld ldg) are overwritten.  On return T
 // (k x k, ld ldt, device) holds R^-1 Q and lam_host[0..k) the ascending Ritz
 // values.  Synchronises the stream once (to read info codes and lam).
+//
+// When the overlap matrix is not numerically positive definite (a block whose
+// columns are nearly dependent, as the Ritz vectors of a Lanczos basis without
+// reorthogonalisation can be), the step is repeated from saved copies of S
+// and Gy with S + shift * max_i S_ii I, shift = 1e-13, 1e-11, 1e-9: the
+// dependent directions then get large Ritz values and leave the wanted lower
+// end of the spectrum, which the reference's Householder QR leaves in place.
 int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, double* T, int ldt, double* lam_dev,
                 double* lam_host) {
   DenseCtx* c = static_cast<DenseCtx*>(ctx);
@@ -553,6 +582,21 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   PC_CHECK_ARG(k >= 1 && k <= c->kmax, "k out of range");
   PC_CHECK_ARG(lds >= k && ldg >= k && ldt >= k, "leading dimension too small");
   const double one = 1.0;
+  PC_HIP(hipMemcpy2DAsync(c->S0, k * sizeof(double), S, lds * sizeof(double), k * sizeof(double), k,
+                          hipMemcpyDeviceToDevice, c->stream));
+  PC_HIP(hipMemcpy2DAsync(c->G0, k * sizeof(double), Gy, ldg * sizeof(double), k * sizeof(double), k,
+                          hipMemcpyDeviceToDevice, c->stream));
+  constexpr int kShiftTries = 3;
+  for (int shift_try = 0;; ++shift_try) {
+  if (shift_try > 0) {
+    PC_HIP(hipMemcpy2DAsync(S, lds * sizeof(double), c->S0, k * sizeof(double), k * sizeof(double), k,
+                            hipMemcpyDeviceToDevice, c->stream));
+    PC_HIP(hipMemcpy2DAsync(Gy, ldg * sizeof(double), c->G0, k * sizeof(double), k * sizeof(double), k,
+                            hipMemcpyDeviceToDevice, c->stream));
+    const double shift = shift_try == 1 ? 1e-13 : (shift_try == 2 ? 1e-11 : 1e-9);
+    hipLaunchKernelGGL(add_diag_shift, dim3(1), dim3(256), 0, c->stream, S, k, lds, shift);
+    PC_LAUNCH_CHECK();
+  }
   hipLaunchKernelGGL(zero_info, dim3(1), dim3(64), 0, c->stream, c->info);
   PC_LAUNCH_CHECK();
   PC_RB(rocsolver_dpotrf(c->h, rocblas_fill_upper, k, S, lds, c->info));
@@ -627,9 +671,15 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
     }
     break;
   }
+  const int i0 = (int)c->host_out[k];
+  if (i0 != 0 && shift_try < kShiftTries) continue;
+  if (shift_try > 0) c->shifted += 1;
+  break;
+  }
   const int i0 = (int)c->host_out[k], i1 = (int)c->host_out[k + 1], i2 = (int)c->host_out[k + 2];
   if (i0 != 0) {
-    pc::set_error("pc_dense_rr: overlap matrix not positive definite (potrf info " + std::to_string(i0) + ")");
+    pc::set_error("pc_dense_rr: overlap matrix not positive definite (potrf info " + std::to_string(i0) +
+                  ") even with a shift of 1e-9 of its largest diagonal entry");
     return pc::PC_ERR_SOLVER;
   }
   if (i1 != 0 || i2 != 0) {
@@ -669,6 +719,15 @@ int pc_dense_trd_control(void* ctx, int spins, int reset, int* out_aborts) {
     c->mwg_ok = true;
     c->mwg_aborts = 0;
   }
+  return pc::PC_OK;
+}
+
+// Number of Rayleigh-Ritz steps of this context that needed the shifted
+// Cholesky factor of the overlap matrix.
+int pc_dense_shift_count(void* ctx, int* out) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c && out, "null pointer");
+  *out = c->shifted;
   return pc::PC_OK;
 }
 

@@ -8,6 +8,8 @@ Mirrors the reference functions (same arguments, same algorithms):
 * ``cheb_filter_slim``-- ``src/Eigensolvers/first_filt.py:78-103``
 * ``first_filt``      -- ``src/Eigensolvers/first_filt.py:6-75``
 * ``chefsi1``         -- ``src/Eigensolvers/chefsi1.py:10-69``
+* ``lanczos``         -- ``src/Eigensolvers/lanczos.py:9-223`` (diagmeth 0/1)
+* ``chsubsp``         -- ``src/Eigensolvers/chsubsp.py:10-91`` (diagmeth 2)
 
 Orthonormalisation + Rayleigh-Ritz (the reference's QR, G = W^T H W, eigh,
 W Q) is done as S = Y^T Y, G_y = Y^T (H Y) (MFMA Gram), FP64 Cholesky /
@@ -28,6 +30,15 @@ import torch
 from . import _lib
 from .device import F64, Scratch, ptr, stream
 from .hamiltonian import BlockLinalg, Hamiltonian, VecOps
+
+
+def _tridiag(alpha, beta):
+    m = len(alpha)
+    t = np.diag(np.asarray(alpha, dtype=np.float64))
+    if m > 1:
+        b = np.asarray(beta[:m - 1], dtype=np.float64)
+        t += np.diag(b, 1) + np.diag(b, -1)
+    return t
 
 
 class SeedStream:
@@ -243,6 +254,162 @@ class Eigensolver:
         self.last_bounds = (lam1, lowerb, upperb)
         Y = self.ch_filter(W, k, deg, lam1, lowerb, upperb)
         return self.rayleigh_ritz(Y, k)
+
+    # -------------------------------------------------------------- Lanczos (diagmeth 0/1) and chsubsp (2)
+    def _lanczos_run(self, m: int, nev_test: int, conv_test: bool, tol: float):
+        """m steps of plain Lanczos (no reorthogonalisation) from a normal random
+        start, every vector kept (FP64 rows of VV).  With ``conv_test`` the run
+        stops when the sum of the nev_test lowest Ritz values of T_k, checked
+        every 10 steps from step min(8 nev_test, m) on, changes by less than tol
+        relative (src/Eigensolvers/lanczos.py:64-131; the reference runs this
+        test for n <= 35000 only).  Returns (VV, alpha, beta, k)."""
+        n, ld = self.n, self.ld
+        VV = torch.empty((m + 1, ld), dtype=F64, device="cuda")
+        coef = self.scratch.get("lzf_coef", 2 * m + 2)
+        part = self.scratch.get("lz_part", 2 * self.lz_nblocks)
+        rand_block(VV[0:1], n, 1, self.seeds.next(), normal=True)
+        _lib.call("pc_lanczos_start", n, ptr(VV[0:1]), ptr(part), stream())
+        zero = self.lz[0:1]
+        zero.zero_()
+        ntest = min(8 * nev_test, m)
+        tr_prev = None
+        k = 0
+        for j in range(m):
+            v0 = VV[j - 1:j] if j > 0 else zero
+            v1, w = VV[j:j + 1], VV[j + 1:j + 2]
+            self._h64_step(v1, None, w, 0.0)                        # w = H v1
+            _lib.call("pc_lanczos_step", n, ptr(v0), ptr(v1), ptr(w), ptr(coef), j, ptr(part), stream())
+            k = j + 1
+            if conv_test and ((k >= ntest and k % 10 == 0) or k == m):
+                c = coef[:2 * k].cpu().numpy()
+                rr = np.linalg.eigvalsh(_tridiag(c[0:2 * k:2], c[1:2 * k - 1:2]))
+                tr = float(np.sum(rr[:nev_test]))
+                if tr_prev is not None and abs(tr - tr_prev) < tol * abs(tr_prev):
+                    break
+                tr_prev = tr
+        c = coef[:2 * k].cpu().numpy()
+        return VV, c[0:2 * k:2], c[1:2 * k:2], k
+
+    def _rr_wide(self, Y: torch.Tensor, kk: int, keep: int):
+        """Rayleigh-Ritz of the kk-vector FP64 block Y (kk may exceed the
+        solver's block width); returns (W (keep, ld) in the solver's dtype, the
+        keep lowest Ritz values), or None when Y spans fewer than keep
+        independent directions.
+
+        The Ritz vectors of a Lanczos run without reorthogonalisation contain
+        near-copies (ghosts), so Y is orthonormalised first, by the eigen-
+        decomposition of its scaled Gram matrix D^-1/2 Y^T Y D^-1/2 = U L U^T
+        (SVQB): directions with L_i below 1e-10 max L are dropped and
+        Y1 = Y D^-1/2 U L^-1/2 has orthonormal columns to rounding.  This plays
+        the part of the reference's Householder QR (src/Eigensolvers/
+        lanczos.py:194), which turns the dependent columns into normalised
+        rounding noise whose Ritz values lie far above the wanted ones."""
+        la = BlockLinalg(self.n, kk)
+        try:
+            S = torch.empty(kk * kk, dtype=F64, device="cuda")
+            la.gram(Y, Y, kk, kk, S, sym=True)
+            Sm = S.view(kk, kk)
+            d = torch.rsqrt(torch.diagonal(Sm).clamp_min(1e-300))
+            Sm.mul_(d[:, None] * d[None, :])
+            ev = np.zeros(kk)
+            _lib.call("pc_dense_syevd", la.dense, kk, ptr(S), kk, ev.ctypes.data_as(C.POINTER(C.c_double)))
+            idx = np.nonzero(ev >= 1e-10 * ev.max())[0]
+            r = len(idx)
+            if r < keep:
+                return None
+            # column j of U is row j of Sm (column-major storage)
+            sel = torch.from_numpy(idx).to("cuda")
+            T1 = Sm.index_select(0, sel) * d[None, :] * torch.from_numpy(ev[idx] ** -0.5).to("cuda")[:, None]
+            T1 = T1.contiguous().view(-1)
+            Y1 = torch.empty((r, self.ld), dtype=F64, device="cuda")
+            la.rotate(Y, T1, kk, r, Y1)
+            HY = torch.empty_like(Y1)
+            dt = self.H.dtype
+            self.H.dtype = F64
+            try:
+                self.H.apply(Y1, HY, r)
+            finally:
+                self.H.dtype = dt
+            S1 = torch.empty(r * r, dtype=F64, device="cuda")
+            G1 = torch.empty(r * r, dtype=F64, device="cuda")
+            la.gram(Y1, Y1, r, r, S1, sym=True)
+            la.gram(Y1, HY, r, r, G1, sym=True)
+            T, lam = la.dense_rr(S1, G1, r)
+            la.rotate(Y1, T[:r * keep], r, keep, HY)          # the first keep columns of T
+        finally:
+            la.close()
+        W = self.bufs[0]
+        if W.dtype == F64:
+            W[:keep].copy_(HY[:keep])
+        else:
+            _lib.call("pc_convert_f64_to_f32", self.n, keep, ptr(HY), HY.shape[1], ptr(W), W.shape[1], stream())
+        self.stats["rr"] += 1
+        return W, np.asarray(lam[:keep])
+
+    def _lanczos_ritz(self, VV, alpha, beta, k: int, nev: int):
+        """Ritz vectors of the Lanczos basis the reference way
+        (src/Eigensolvers/lanczos.py:149-209): eigenpairs of T_k, the count of
+        distinct Ritz values (within 1e-5) sets mev = min(nev * min(1.8^(k /
+        distinct), 3.2), k) vectors V Y, which are orthonormalised and
+        Rayleigh-Ritz projected; the nev lowest pairs are returned, with the
+        bound max|theta| + beta_k.  Should the mev vectors span fewer than nev
+        directions, mev is doubled (up to k)."""
+        rr, X1 = np.linalg.eigh(_tridiag(alpha[:k], beta[:k - 1]))
+        distinct = [rr[0]]
+        for x in rr[1:]:
+            if all(abs(x - d) >= 1e-5 for d in distinct[-8:]):
+                distinct.append(x)
+        mult = min(1.8 ** (len(rr) / len(distinct)), 3.2)
+        mev = max(nev, int(min(nev * mult, k)))
+        bound = float(np.max(np.abs(rr)) + beta[k - 1])
+        while True:
+            Q = torch.tensor(np.asfortranarray(X1[:, :mev]).reshape(-1, order="F"), dtype=F64, device="cuda")
+            Y = torch.empty((mev, self.ld), dtype=F64, device="cuda")
+            la = BlockLinalg(self.n, max(k, mev))
+            la.rotate(VV, Q, k, mev, Y)
+            la.close()
+            out = self._rr_wide(Y, mev, nev)
+            if out is not None:
+                self.last_lanczos = {"steps": k, "mev": mev, "distinct": len(distinct)}
+                return out[0], out[1], bound
+            if mev >= k:
+                raise _lib.ParsecHipError(f"lanczos: {k} steps span fewer than {nev} independent Ritz vectors")
+            mev = min(2 * mev, k)
+
+    def lanczos(self, k: int, m: int, tol: float = 1e-5):
+        """The reference's ``lanczos(B, nev, v, m, tol)`` (diagmeth 0 at the
+        first SCF iteration, diagmeth 1 every iteration): m Lanczos steps, k
+        Ritz pairs (src/Eigensolvers/lanczos.py:9-223)."""
+        VV, a, b, kd = self._lanczos_run(m, k, self.n <= 35000, tol)
+        W, lam, _ = self._lanczos_ritz(VV, a, b, kd, k)
+        return W, lam
+
+    def chsubsp(self, k: int, deg: int):
+        """The reference's ``chsubsp(deg, nev, H)`` (diagmeth 2, first SCF
+        iteration; src/Eigensolvers/chsubsp.py:10-91): Lanczos with max(3k, 450)
+        steps (src/Eigensolvers/lanczosForChsubsp.py:7-193) for the start block,
+        its Ritz values and the upper bound; then up to 30 passes of the scaled
+        Chebyshev filter on [max ritz, bound] + Rayleigh-Ritz until the sum of the
+        k Ritz values changes by less than 5 %."""
+        steps = max(3 * k, 450)
+        VV, a, b, kd = self._lanczos_run(steps, k, self.n <= 35000, 1e-5)
+        W, ritzv, upperb = self._lanczos_ritz(VV, a, b, kd, k)
+        del VV
+        tr0 = float(np.sum(ritzv))
+        outer = 0
+        for _ in range(30):
+            outer += 1
+            lam1, lowerb = float(np.min(ritzv)), float(np.max(ritzv))
+            if lowerb > upperb:
+                raise ValueError("bounds are wrong")
+            Y = self.ch_filter(W, k, deg, lam1, lowerb, upperb)
+            W, ritzv = self.rayleigh_ritz(Y, k)
+            tr1 = float(np.sum(ritzv[:k]))
+            if abs(tr1 - tr0) < 0.05 * abs(tr1):
+                break
+            tr0 = tr1
+        self.last_chsubsp = {"lanczos_steps": kd, "outer_iterations": outer, "upperb": upperb}
+        return W, ritzv
 
     def close(self):
         self.la.close()

@@ -33,7 +33,7 @@ from .eigensolvers import Eigensolver
 from .grid import Grid, make_grid, reference_fd_radius
 from .hamiltonian import Hamiltonian, leading_dim
 from .hartree import HartreeSolver
-from .mixer import DeviceMsecant1
+from .mixer import make_mixer
 from .occupations import occupations
 
 RY_EV = 13.605698066
@@ -89,6 +89,15 @@ class SCFConfig:
     # 1-4) this takes the mixed solve from 17 to 15 SCF iterations; the energy
     # at the stop stays within the spread the unweighted start shows.
     weighted_start: bool = True
+    # eigensolver schedule (src/rsdft_solver.py:289-316): 3 = first_filt then
+    # chefsi1 (default); 2 = chsubsp then chefsi1; 1 = Lanczos every
+    # iteration; 0 = Lanczos then chefsi1
+    diagmeth: int = 3
+    # potential mixer (src/Mixer/mixer.py): "msecant1" (default), "msecant2", "msecant3"
+    mixer: str = "msecant1"
+    # system charge: electrons removed (src/rsdft_solver.py:224-226,253-255 scale
+    # rho0 and V_H0 by (Z - charge)/Z and take charge off the electron count)
+    z_charge: float = 0.0
     verbose: bool = False
 
 
@@ -131,6 +140,14 @@ class SCFProblem:
         self.rho0, self.hpot0, self.ppot, self.diag_info = ionic.pseudo_diag(self.grid, self.H.dgrid, self.atoms)
         self.e_nuc = ionic.nuclear(self.atoms)
         self.nelec = ionic.nelectrons(self.atoms)
+        if cfg.z_charge != 0:
+            zelec = self.nelec
+            if zelec - cfg.z_charge < 0:
+                raise ValueError("Problem with charge state. Negative number of electrons.")
+            scale = (zelec - cfg.z_charge) / zelec
+            self.rho0 = self.rho0 * scale
+            self.hpot0 = self.hpot0 * scale
+            self.nelec = zelec - cfg.z_charge
         self.n_atoms = sum(x.shape[0] for _, x in self.atoms)
         torch.cuda.synchronize()
         self.setup_time = time.perf_counter() - t0
@@ -146,10 +163,16 @@ def _xc(n, rho, vxc, part, out1):
 
 def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: torch.Tensor | None = None,
             max_wall: float | None = None, warmup: int | None = None,
-            start_weight_rho: torch.Tensor | None = None) -> SCFResult:
+            start_weight_rho: torch.Tensor | None = None, callback=None) -> SCFResult:
     """Run the SCF loop.  ``rho_init``/``hpot_init`` replace (rho0, V_H0) for a
     learned start, as the reference's ML density path does
     (``src/V_ion/pseudoDiag_ML4Den_poisson.py:392-540``).
+
+    ``callback(record)``, if given, is called after every iteration with the
+    iteration number, diagonalisation label and time, eigenvalues and
+    occupations, Hartree CG iterations and time, SCF error and energies (what
+    the reference writes to its log per iteration) and the new density
+    (``rho``, e/bohr^3, the loop's own buffer: read it inside the callback).
 
     ``start_weight_rho``: the density whose square root weights first_filt's
     random start vectors (``cfg.weighted_start``); default ``rho_init`` when
@@ -200,7 +223,10 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     dense_method = {"syevd": 0, "syevj": 1, "tridiag": 2}[cfg.dense_method]
     _lib.call("pc_dense_set_method", eig.la.dense, dense_method)
     hart = HartreeSolver(H.dgrid, h, reference_aliasing=cfg.pcg_aliasing, prec=cfg.pcg_prec)
-    mixer = DeviceMsecant1(n, cfg.maxits + 2)
+    mixer = make_mixer(cfg.mixer, n, cfg.maxits + 2)
+    if cfg.diagmeth not in (0, 1, 2, 3):
+        raise ValueError("diagmeth must be 0, 1, 2 or 3")
+    lanczos_steps = nev + 500
 
     err = 10.0 + cfg.tol
     its = 0
@@ -218,7 +244,14 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         its += 1
         t0 = time.perf_counter()
         H.set_potential(pot)
-        if its == 1:
+        if cfg.diagmeth == 1 or (its == 1 and cfg.diagmeth == 0):
+            W, lam = eig.lanczos(k, lanczos_steps, 1e-5)
+            diag_label = f"lanczos_gpu (diagmeth={cfg.diagmeth})"
+        elif its == 1 and cfg.diagmeth == 2:
+            W, lam = eig.chsubsp(k, cfg.poldeg)
+            diag_label = "chsubsp_gpu (diagmeth=2)"
+        elif its == 1:
+            diag_label = "first_filt_gpu (diagmeth=3)"
             sw = None
             wrho = start_weight_rho if start_weight_rho is not None else rho_init
             if wrho is not None and cfg.weighted_start:
@@ -240,6 +273,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
             if cfg.polish_deg is not None and H.dtype == F64 and prob.dtype == F32:
                 deg = cfg.polish_deg
             W, lam = eig.chefsi1(W, lam, deg, nev)
+            diag_label = f"chebsf_gpu (diagmeth={cfg.diagmeth})"
         t1 = time.perf_counter()
         _, occ = occupations(lam[:nev], cfg.fermi_temp, prob.nelec, 1e-6)
         occ2_dev[:nev].copy_(torch.from_numpy(2.0 * np.asarray(occ, dtype=np.float64)))
@@ -264,6 +298,10 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
         hist.append({"iter": its, "err": err, "total_ry": e["total_ry"], "dE_ha_atom": de, "cg_its": cg_its,
                      "cg_rel": hart.last_rel, "bounds": getattr(eig, "last_bounds", None),
                      "lam": np.array(lam[:nev])})
+        if callback is not None:
+            callback({"iter": its, "diag_label": diag_label, "diag_time": t1 - t0, "lam": np.array(lam[:nev]),
+                      "occ": np.array(occ[:nev]), "cg_its": cg_its, "hart_time": t2 - t1, "err": err,
+                      "energies": dict(e), "precision": "fp64" if H.dtype == F64 else "fp32", "rho": rho})
         # pot <- mixer(pot, pot_new - pot)
         f1.copy_(pot_new)
         _lib.call("pc_lincomb", n, None, n, 0, None, 1.0, ptr(pot), -1.0, ptr(f1), stream())

@@ -323,6 +323,102 @@ def test_msecant1_is_dense_broyden1_and_meets_the_secant_equation(seed):
         x = x_new
 
 
+def _dense_broyden2_inverse(B, dx, df):
+    """Broyden's second ("bad") method in inverse form for x+ = x + B f:
+    B+ = B - (dx + B df) df^T / (df^T df), the rank-one change of B of least
+    Frobenius norm with B+ df = -dx."""
+    return B - np.outer(dx + B @ df, df) / (df @ df)
+
+
+def _nonlinear_residual(seed, n):
+    rng = np.random.default_rng(seed)
+    J = np.eye(n) + 0.3 * rng.standard_normal((n, n))
+    xs = rng.standard_normal(n)
+    return rng, (lambda x: -J @ (x - xs) - 0.05 * np.tanh(x - xs))
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_msecant2_is_dense_broyden2(seed):
+    """src/Mixer/msecant2.py:24-92 (group_size 1, EN_like 0, no restart):
+    every update equals the dense inverse Broyden-II matrix from B0 = mix*I and
+    meets the secant equation."""
+    from oracle.mixer import Msecant2
+    n, mix = 14, 0.5
+    rng, f = _nonlinear_residual(seed, n)
+    mx = Msecant2(mix=mix, restart_factor=0.0)
+    B = mix * np.eye(n)
+    x = rng.standard_normal(n)
+    xp = fp = None
+    for it in range(7):
+        fx = f(x)
+        x_new, m = mx(x, fx)
+        if xp is not None:
+            dx, df = x - xp, fx - fp
+            B_old = B
+            B = _dense_broyden2_inverse(B_old, dx, df)
+            Brec = mix * np.eye(n) - sum(np.outer(a, b) for a, b in zip(mx.DX[:-1], mx.DF[:-1]))
+            assert np.allclose(Brec, B, rtol=1e-10, atol=1e-12)
+            assert np.allclose(B @ df, -dx, rtol=1e-10, atol=1e-12)
+            # the Type-I update of the same pair is a different matrix, so the check tells them apart
+            assert not np.allclose(B, _dense_broyden1_inverse(B_old, dx, df), rtol=1e-6)
+        assert np.allclose(x_new, x + B @ fx, rtol=1e-10, atol=1e-12)
+        xp, fp = x.copy(), fx.copy()
+        x = x_new
+
+
+def test_msecant2_restart_and_stage_alternation():
+    """After a restart msecant2 keeps (x1, f1) and enters stage 2 (no new pair
+    stored), then stage 1 (applies the stored pairs without updating them)."""
+    from oracle.mixer import Msecant2
+    mx = Msecant2()
+    x0, f0 = np.ones(3), np.full(3, 1e-3)
+    mx(x0, f0)
+    mx(x0 + 1, f0 * 2)
+    xr, m = mx(x0 + 2, np.full(3, 10.0))
+    assert m == 0 and np.allclose(xr, (x0 + 2) + 0.5 * 10.0) and mx.stage == 2 and len(mx.DX) == 1
+    mx(x0 + 3, np.full(3, 9.0))
+    assert mx.stage == 1 and len(mx.DX) == 1
+    mx(x0 + 4, np.full(3, 8.0))
+    assert mx.stage == 2 and len(mx.DX) == 2
+
+
+@pytest.mark.parametrize("seed", [2, 3, 4])
+def test_msecant3_picks_type_by_its_rule_and_matches_dense_update(seed):
+    """src/Mixer/msecant3.py:45-150: each update is the dense Broyden-I or
+    Broyden-II change, the one the secant-error rule picks (Type I when
+    |df.df_prev| |N.df| > |df.df| |dx.dx_prev|, N = -B^T dx), Type I first."""
+    from oracle.mixer import Msecant3
+    n, mix = 12, 0.5
+    rng, f = _nonlinear_residual(seed, n)
+    mx = Msecant3(mix=mix, restart_factor=0.0)
+    B = mix * np.eye(n)
+    x = rng.standard_normal(n)
+    xp = fp = dxp = dfp = None
+    types = []
+    for it in range(8):
+        fx = f(x)
+        x_new, m = mx(x, fx)
+        if xp is not None:
+            dx, df = x - xp, fx - fp
+            if dxp is None:
+                want = 1
+            else:
+                nv = -B.T @ dx
+                lhs = abs(df @ dfp) * abs(nv @ df)
+                rhs = abs(df @ df) * abs(dx @ dxp)
+                want = 1 if lhs > rhs else 2
+            assert mx.last_type == want
+            types.append(want)
+            B = _dense_broyden1_inverse(B, dx, df) if want == 1 else _dense_broyden2_inverse(B, dx, df)
+            Brec = mix * np.eye(n) - sum(np.outer(a, b) for a, b in zip(mx.DX[:-1], mx.DF[:-1]))
+            assert np.allclose(Brec, B, rtol=1e-9, atol=1e-11)
+            dxp, dfp = dx, df
+        assert np.allclose(x_new, x + B @ fx, rtol=1e-9, atol=1e-11)
+        xp, fp = x.copy(), fx.copy()
+        x = x_new
+    assert types[0] == 1
+
+
 def test_msecant1_mutation_is_caught():
     """A plausible mistake (the sign of the N correction flipped) breaks the
     secant equation, so the check above pins that term."""
