@@ -607,6 +607,235 @@ int launch_lean(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
   return pc::PC_OK;
 }
 
+// ---- the paired z-march (FP32, rows of 65..128 points) ---------------------
+// stencil_lean with two adjacent x points per lane: lane l holds x = 2l, 2l+1,
+// so one wavefront covers a whole row and every Y / X / V access and the store
+// move 8 bytes per lane (64-bit buffer accesses) instead of 4.  The row data
+// starts at any element, so a pair straddles the chord ends: a pair with at
+// least one point in the chord is loaded whole and its outside point is set to
+// zero at first use (queue entries: when the plane enters the stencil as z+R;
+// halo rows: before they go to LDS), and a store writes the pair when both
+// points are in the chord, else the one point that is (a 32-bit store).  The
+// two pairs that would straddle the ends of the array (flat start -1 or
+// npts-1) are loaded from 0 or npts-2 and shifted into place, so every 64-bit
+// access lies wholly inside the vector.
+typedef float pc_f2 __attribute__((ext_vector_type(2)));
+typedef unsigned pc_u2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ pc_f2 pc_bload2(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(pc_f2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+
+template <int R, int TY, bool HASX>
+__global__ __launch_bounds__(64 * TY)
+void stencil_pair(StArgs g, const float* Y, const float* X, float* Out, const float* V, float ca, float cb, float cc) {
+  using T = float;
+  constexpr int NWV = TY;                     // one wave per row
+  constexpr int LW = 128 + 2 * R;
+  constexpr int LR = TY + 2 * R;
+  constexpr int NHI = 2 * R;                  // halo rows
+  constexpr int HT = (NHI + NWV - 1) / NWV;
+  constexpr int NS = PC_LEAN_NS;
+  constexpr int QL = pc_lean_ql(R);
+  constexpr int ES = 4;
+  constexpr int OOB = 0x7ffffff0;
+  __shared__ __align__(16) T tile[2][LR * LW];
+  extern __shared__ int2 mtab[];
+
+  const int bid = blockIdx.x;
+  const int L = (bid & 7) * g.nb8 + (bid >> 3);
+  if (L >= g.nblocks) return;
+  const int v = L / g.nslab;
+  const int slab = L - v * g.nslab;
+  const int j0 = slab * TY;
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int ny = g.ny, nz = g.nz;
+
+  int zb = nz, ze = 0;
+  for (int t = 0; t < TY; ++t) {
+    const int j = j0 + t;
+    if (j < ny) {
+      zb = min(zb, g.zrange[2 * j]);
+      ze = max(ze, g.zrange[2 * j + 1]);
+    }
+  }
+  if (zb >= ze) return;
+  const int zlo = zb - R;
+  const int zm = (ze - zb + QL - 1) / QL * QL + QL;
+
+  const unsigned nbytes = (unsigned)(g.npts * ES);
+  const long long base = (long long)v * g.ld;
+  const __amdgpu_buffer_rsrc_t ry = pc_rsrc(Y + base, nbytes);
+  const __amdgpu_buffer_rsrc_t rx = pc_rsrc(HASX ? X + base : Y + base, nbytes);
+  const __amdgpu_buffer_rsrc_t ro = pc_rsrc(Out + base, nbytes);
+  const __amdgpu_buffer_rsrc_t rv = pc_rsrc(V, nbytes);
+
+  for (int t = threadIdx.x; t < 2 * LR * LW; t += 64 * NWV) (&tile[0][0])[t] = T(0);
+  for (int e = threadIdx.x; e < LR * zm; e += 64 * NWV) {
+    const int t = e / zm, z = zlo + (e - t * zm);
+    const int j = j0 - R + t;
+    mtab[e] = (j >= 0 && j < ny && z >= 0 && z < nz) ? g.meta[j + ny * z] : make_int2(0, 0);
+  }
+  __syncthreads();
+
+  auto mrow = [&](int t, int z) -> int2 {
+    const int2 m = mtab[t * zm + (z - zlo)];
+    return make_int2(__builtin_amdgcn_readfirstlane(m.x), __builtin_amdgcn_readfirstlane(m.y));
+  };
+  const int i0 = 2 * lane;
+  const int last2 = (int)g.npts - 2;           // the host checks npts >= 2
+  // flat start of the pair (m.x + i0) clamped to [0, npts-2]
+  auto pair_start = [&](int2 m) -> int { return min(max(m.x + i0, 0), last2); };
+  // byte offset of the pair: out of range unless a point is in the chord
+  auto off_pair = [&](int2 m) -> int {
+    const int lo = m.y & 0xffff, len = (m.y >> 16) - lo;
+    const bool any = (unsigned)(i0 - lo) < (unsigned)len || (unsigned)(i0 + 1 - lo) < (unsigned)len;
+    return any ? pair_start(m) * ES : OOB;
+  };
+  // the pair loaded from pair_start(m), shifted back to x = i0, i0+1, with the
+  // points outside the chord set to zero
+  auto fix_pair = [&](int2 m, pc_f2 a) -> pc_f2 {
+    const int start = m.x + i0, st = pair_start(m);
+    const int lo = m.y & 0xffff, len = (m.y >> 16) - lo;
+    T x0 = start > st ? a.y : (start < st ? T(0) : a.x);
+    T x1 = start < st ? a.x : (start > st ? T(0) : a.y);
+    pc_f2 r;
+    r.x = (unsigned)(i0 - lo) < (unsigned)len ? x0 : T(0);
+    r.y = (unsigned)(i0 + 1 - lo) < (unsigned)len ? x1 : T(0);
+    return r;
+  };
+
+  const int town = R + wv;
+  int th[HT];
+#pragma unroll
+  for (int u = 0; u < HT; ++u) {
+    const int k = wv + NWV * u;
+    th[u] = k < R ? k : k + TY;
+  }
+  auto has_halo = [&](int u) { return wv + NWV * u < NHI; };
+
+  pc_f2 q[QL];
+  pc_f2 xq[NS], vq[NS], hq[NS][HT];
+  // prefill planes zb-R .. zb-R+QL-1; the 2R planes zb-R .. zb+R-1 are used
+  // before the march masks anything, so they are masked here
+#pragma unroll
+  for (int d = 0; d < QL; ++d) q[d] = pc_bload2(ry, off_pair(mrow(town, zb - R + d)));
+#pragma unroll
+  for (int d = 0; d < 2 * R; ++d) q[d] = fix_pair(mrow(town, zb - R + d), q[d]);
+
+  auto issue_plane = [&](auto set, int z) {
+    constexpr int S = decltype(set)::value;
+    const int po = pair_start(mrow(town, z)) * ES;
+    vq[S] = pc_bload2(rv, po);
+    if constexpr (HASX) xq[S] = pc_bload2(rx, po);
+#pragma unroll
+    for (int u = 0; u < HT; ++u)
+      if (has_halo(u)) hq[S][u] = pc_bload2(ry, off_pair(mrow(th[u], z)));
+  };
+  pc_unroll_seq(std::make_integer_sequence<int, NS - 1>{}, [&](auto k) {
+    constexpr int K = decltype(k)::value;
+    issue_plane(std::integral_constant<int, K>{}, zb + K);
+  });
+
+  const T w0 = T(g.w[0]);
+  T wd[R + 1];
+#pragma unroll
+  for (int d = 1; d <= R; ++d) wd[d] = T(g.w[d]);
+
+  auto plane = [&](auto phase, int z) {
+    constexpr int P = decltype(phase)::value;
+    constexpr int S = P % NS;
+    constexpr int B = P & 1;
+    issue_plane(std::integral_constant<int, (P + NS - 1) % NS>{}, z + NS - 1);
+    __builtin_amdgcn_sched_barrier(0);
+    // plane z+R enters the stencil: shift and zero its points outside the chord
+    q[(P + 2 * R) % QL] = fix_pair(mrow(town, z + R), q[(P + 2 * R) % QL]);
+    T* tb = tile[B];
+    {
+      const pc_f2 c = q[(P + R) % QL];
+      tb[town * LW + R + i0] = c.x;
+      tb[town * LW + R + i0 + 1] = c.y;
+    }
+#pragma unroll
+    for (int u = 0; u < HT; ++u) {
+      if (has_halo(u)) {
+        const pc_f2 h = fix_pair(mrow(th[u], z), hq[S][u]);
+        tb[th[u] * LW + R + i0] = h.x;
+        tb[th[u] * LW + R + i0 + 1] = h.y;
+      }
+    }
+    lds_barrier();
+    const int2 mo = mrow(town, z);
+    const T* trow = tb + town * LW + R + i0;
+    T xs[2 * R + 2];                           // own row at x = i0-R .. i0+R+1
+#pragma unroll
+    for (int d = 0; d < 2 * R + 2; ++d) xs[d] = trow[d - R];
+    const pc_f2 yc = q[(P + R) % QL];
+    pc_f2 acc = w0 * yc;
+#pragma unroll
+    for (int d = 1; d <= R; ++d) {
+      // the summation order of stencil_lean: x-, x+, y-, y+, z-, z+
+      const pc_f2 zm = q[(P + R - d) % QL], zp = q[(P + R + d) % QL];
+      pc_f2 nb;
+      nb.x = xs[R - d] + xs[R + d] + trow[-d * LW] + trow[d * LW] + zm.x + zp.x;
+      nb.y = xs[R + 1 - d] + xs[R + 1 + d] + trow[-d * LW + 1] + trow[d * LW + 1] + zm.y + zp.y;
+      acc += wd[d] * nb;
+    }
+    pc_f2 o = acc + (ca * fix_pair(mo, vq[S]) + cb) * yc;
+    if constexpr (HASX) o += cc * fix_pair(mo, xq[S]);
+    {
+      const int lo = mo.y & 0xffff, len = (mo.y >> 16) - lo;
+      const bool in0 = (unsigned)(i0 - lo) < (unsigned)len;
+      const bool in1 = (unsigned)(i0 + 1 - lo) < (unsigned)len;
+      const int o2 = (in0 && in1) ? (mo.x + i0) * ES : OOB;
+      const int o1 = (in0 != in1) ? (mo.x + i0 + (in0 ? 0 : 1)) * ES : OOB;
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pc_u2, o), ro, o2, 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, in0 ? o.x : o.y), ro, o1, 0, 0);
+    }
+    q[P] = pc_bload2(ry, off_pair(mrow(town, z - R + QL)));
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  for (int z0 = zb; z0 < ze; z0 += QL) {
+    pc_unroll_seq(std::make_integer_sequence<int, QL>{}, [&](auto ph) {
+      constexpr int P = decltype(ph)::value;
+      plane(ph, z0 + P);
+    });
+  }
+}
+
+#ifndef PC_STENCIL_PAIR
+#define PC_STENCIL_PAIR 1
+#endif
+
+template <int R>
+int launch_pair(const GridDev* gd, const float* Y, const float* X, float* Out, const float* V, int nvec, long long ld,
+                double scale, double ca, double cb, double cc, hipStream_t st) {
+  constexpr int TY = PC_LEAN_TY;
+  constexpr int QL = pc_lean_ql(R);
+  StArgs a;
+  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
+  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
+  a.ld = ld;
+  a.nslab = (gd->ny + TY - 1) / TY;
+  a.nvec = nvec;
+  a.nblocks = a.nslab * nvec;
+  a.nb8 = (a.nblocks + 7) / 8;
+  for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
+  a.w[0] = 3.0 * gd->w[0] * scale;
+  for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
+  const dim3 grid(8 * a.nb8), block(64 * TY);
+  const size_t mbytes = sizeof(int2) * (size_t)(TY + 2 * R) * (size_t)(gd->nz + 2 * QL);
+  if (X)
+    hipLaunchKernelGGL((stencil_pair<R, TY, true>), grid, block, mbytes, st, a, Y, X, Out, V, float(ca), float(cb),
+                       float(cc));
+  else
+    hipLaunchKernelGGL((stencil_pair<R, TY, false>), grid, block, mbytes, st, a, Y, X, Out, V, float(ca), float(cb),
+                       float(cc));
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
 template <typename T, int R, int SX, int TYW, int NV, int NB = 2, int NW = 4>
 constexpr int zmarch_lds_bytes() {
   return NB * NV * (NW * TYW + 2 * R) * (64 * SX + 2 * R) * (int)sizeof(T);
@@ -675,6 +904,20 @@ int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
   constexpr int NW = PC_STENCIL_NW;
   if (dot_part)   // per-vector partial sums need one vector per workgroup
     return launch_cfg<T, R, SX, 1, 1, 2, 1, 4>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, dot_part, st);
+#if PC_STENCIL_PAIR
+  // Measured on MI355X (one FP32 recurrence step, lean vs paired, identical
+  // results): Si87H76 sphere nx 106, 275 vectors, radius 4: 801 vs 881 us;
+  // radius 3: 716 vs 880 us; C10H8 sphere nx 92, 49 vectors: 133 vs 178 us;
+  // sphere nx 75, radius 1: 46 vs 72 us; 128^3 sphere, 256 vectors, radius 6:
+  // 1539 vs 1430 us.  The paired kernel needs 89 VGPRs at radius 4 (5 waves
+  // per SIMD against 8), which costs more than the wider accesses gain,
+  // except at radius 6, where the lean kernel's queue is long too.
+  if constexpr (SX == 2 && sizeof(T) == 4 && R >= 6)
+    if (gd->npts >= 2)
+      return launch_pair<R>(gd, reinterpret_cast<const float*>(Y), reinterpret_cast<const float*>(X),
+                          reinterpret_cast<float*>(Out), reinterpret_cast<const float*>(V), nvec, ld, scale, ca, cb,
+                          cc, st);
+#endif
 #ifndef PC_STENCIL_NO_LEAN
   if constexpr (SX <= 2) return launch_lean<T, R, SX>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, st);
 #endif

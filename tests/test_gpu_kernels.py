@@ -30,6 +30,10 @@ def _oracle_lap(g, radius):
 CASES = [  # nx, ny, nz, kind, radius
     (20, 20, 20, "cube", 4), (21, 21, 21, "sphere", 4), (18, 18, 18, "sphere", 6), (16, 16, 16, "cube", 1),
     (17, 17, 17, "cube", 2), (19, 19, 19, "sphere", 3), (70, 12, 9, "cube", 4), (130, 9, 7, "cube", 6),
+    # rows of 65..128 points (the FP32 paired kernel): chords starting at odd
+    # and even x, first and last points of the array at both parities
+    (67, 67, 67, "sphere", 3), (75, 75, 75, "sphere", 1), (66, 66, 66, "sphere", 6), (97, 11, 13, "cube", 2),
+    (106, 106, 106, "sphere", 4),
 ]
 
 
