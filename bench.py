@@ -86,7 +86,7 @@ def _log(msg):
 def measured_traffic(name):
     """HBM bytes per launch of the recurrence stencil from the committed
     counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
-    path = os.path.join(ROOT, "profiles", f"stencil_zmarch_{name}_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"stencil_lean_{name}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -264,6 +264,36 @@ def _repeat(prob, reps, warm=1, label="", **kw):
     return res, float(sum(times) / len(times)), cold
 
 
+def _load_net(spec):
+    ckpt = spec
+    if ckpt == "default":
+        from parsec_python_amd.learned import default_checkpoint
+        ckpt = default_checkpoint() if os.path.exists(default_checkpoint()) else "none"
+    if ckpt == "none":
+        return None, ckpt
+    from parsec_python_amd.learned import UNet
+    return UNet.load(ckpt), ckpt
+
+
+def timed_only(args, atoms, dom, nev, etol):
+    """The timed configuration alone (mixed precision, multigrid CG, learned
+    start): `warmup` + `steps` solves, their times on stderr and one JSON line."""
+    from parsec_python_amd.scf import SCFConfig, SCFProblem
+    net, _ = _load_net(args.learned)
+    prob = SCFProblem(atoms, dom, nev, SCFConfig(precision="mixed", tol=1e-12, maxits=60, domain="sphere", seed=1,
+                                                 etol_ha_per_atom=etol, pcg_maxit=PCG_MAXIT, pcg_aliasing=False,
+                                                 pcg_prec="mg"))
+    for _ in range(args.warmup):
+        solve(prob, net=net)
+    times = []
+    for _ in range(args.steps):
+        res, t = solve(prob, net=net)
+        _log(f"mixed solve: {res.iterations} iterations, {t:.3f} s")
+        times.append(t)
+    prob.close()
+    print(json.dumps({"timed_only": True, "solve_s": [round(t, 4) for t in times], "iterations": res.iterations}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -274,6 +304,8 @@ def main():
     ap.add_argument("--learned", default="default",
                     help="trained initialiser checkpoint (.npz); 'default' = the packaged one if present, 'none' = SAD")
     ap.add_argument("--no-reference", action="store_true", help="skip the tightly converged FP64 reference energy")
+    ap.add_argument("--timed-only", action="store_true",
+                    help="run only the timed solves (for profiling them); prints their times, no baselines")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -286,6 +318,8 @@ def main():
 
     atoms, dom, nev = _atoms(args.workload)
     etol = 1e-6
+    if args.timed_only:
+        return timed_only(args, atoms, dom, nev, etol)
     # the baselines and controls: one warm-up solve, then the mean of `reps` solves
     reps = max(1, min(args.steps, 5))
     common = dict(tol=1e-12, maxits=60, domain="sphere", seed=1, etol_ha_per_atom=etol, pcg_maxit=PCG_MAXIT,
@@ -308,14 +342,7 @@ def main():
         del probr
         torch.cuda.empty_cache()
 
-    net = None
-    ckpt = args.learned
-    if ckpt == "default":
-        from parsec_python_amd.learned import default_checkpoint
-        ckpt = default_checkpoint() if os.path.exists(default_checkpoint()) else "none"
-    if ckpt != "none":
-        from parsec_python_amd.learned import UNet
-        net = UNet.load(ckpt)
+    net, ckpt = _load_net(args.learned)
     # the same FP64 solve with the multigrid-preconditioned Hartree CG: separates
     # the preconditioner's share of the speed-up from that of the precision and start
     prob64m = SCFProblem(atoms, dom, nev, SCFConfig(precision="fp64", **dict(common, pcg_prec="mg")))
@@ -395,7 +422,7 @@ def main():
                           "mixed_vs_converged_fp64_ha_per_atom": None if e_ref is None else abs(e_mixed - e_ref),
                           "fp64_stop_vs_converged_fp64_ha_per_atom": None if e_ref is None else abs(e_fp64 - e_ref)},
         "energy_ry": {"mixed": res.energies["total_ry"], "fp64": res64.energies["total_ry"]},
-        "roofline": {"kernel": "stencil_zmarch<float> (fused FD stencil + V + Chebyshev step)", "bound": "hbm",
+        "roofline": {"kernel": "stencil_lean<float> (fused FD stencil + V + Chebyshev step)", "bound": "hbm",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "launches": int(len(durs)), "avg_launch_us": round(1e3 * float(durs.mean()), 2) if len(durs) else None,

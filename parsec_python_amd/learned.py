@@ -422,22 +422,98 @@ def volume_pair(prob, res):
     return x, t, s
 
 
-def iterations_with(geo, net: UNet | None, precision="mixed", etol=1e-6, domain="cube"):
-    """SCF iterations and energy from the SAD start (net None) or the learned start."""
+def iterations_with(geo, net: UNet | None, precision="mixed", etol=1e-6, domain="cube", seed=1,
+                    sad_weighted=False):
+    """SCF iterations and energy from the SAD start (net None; with
+    ``sad_weighted`` the first_filt start vectors are weighted by the SAD
+    density, the control for the learned start's weighting) or the learned
+    start."""
     from . import molecules as M
     from .scf import SCFConfig, SCFProblem, run_scf
     atoms, dom, nev = M.prepare(geo, h_ang=TRAIN_H_ANG, radius_ang=TRAIN_RADIUS_ANG)
     cfg = SCFConfig(tol=1e-12, maxits=60, domain=domain, precision=precision, etol_ha_per_atom=etol,
-                    pcg_maxit=1000, pcg_aliasing=False, seed=1)
+                    pcg_maxit=1000, pcg_aliasing=False, seed=seed)
     prob = SCFProblem(atoms, dom, nev, cfg)
     if net is None:
-        res = run_scf(prob)
+        res = run_scf(prob, start_weight_rho=prob.rho0 if sad_weighted else None)
     else:
         rho1, vh1 = learned_start(prob, net)
         res = run_scf(prob, rho_init=rho1, hpot_init=vh1)
     out = (res.iterations, 0.5 * res.energies["total_ry"] / prob.n_atoms, res.converged)
     prob.close()
     return out
+
+
+def evaluate_main(argv=None):
+    """Held-out evaluation of a checkpoint: for fresh seeded geometries (the
+    training kinds, seeds it never saw) the start density's L1 distance to the
+    converged one and the mixed-precision SCF iterations to |dE| < 1e-6
+    Ha/atom from the SAD start, the SAD start with SAD-weighted start vectors
+    (the control) and the learned start (weighted by the learned density),
+    each for several start-vector seeds; writes a Markdown report."""
+    import argparse
+    import time
+    from . import molecules as M
+    from .scf import SCFConfig, SCFProblem, run_scf
+    ap = argparse.ArgumentParser(prog="python -m parsec_python_amd.learned evaluate")
+    ap.add_argument("--checkpoint", default=None)
+    ap.add_argument("--geometries", type=int, default=8)
+    ap.add_argument("--seeds", type=int, default=3)
+    ap.add_argument("--heldout-seed", type=int, default=777001)
+    ap.add_argument("--out", default="unet_heldout_report.md")
+    a = ap.parse_args(argv)
+    net = UNet.load(a.checkpoint or default_checkpoint())
+    rng = np.random.default_rng(a.heldout_seed)
+    rows = []
+    t0 = time.perf_counter()
+    for g in range(a.geometries):
+        kind = "si" if g % 4 == 3 else "alkane"
+        geo = sample_geometry(kind, rng)
+        atoms, dom, nev = M.prepare(geo, h_ang=TRAIN_H_ANG, radius_ang=TRAIN_RADIUS_ANG)
+        its = {"sad": [], "sad_weighted": [], "learned": []}
+        e = {"sad": [], "sad_weighted": [], "learned": []}
+        l1 = None
+        for sd in range(1, a.seeds + 1):
+            cfg = SCFConfig(tol=1e-12, maxits=60, domain="cube", precision="mixed", etol_ha_per_atom=1e-6,
+                            pcg_maxit=1000, pcg_aliasing=False, seed=sd)
+            prob = SCFProblem(atoms, dom, nev, cfg)
+            r0 = run_scf(prob)
+            r0w = run_scf(prob, start_weight_rho=prob.rho0)
+            rho1, vh1 = learned_start(prob, net)
+            r1 = run_scf(prob, rho_init=rho1, hpot_init=vh1)
+            if l1 is None:
+                rc = r0.rho * prob.h ** 3
+                l1 = (float((prob.rho0 - rc).abs().sum()) / prob.nelec, float((rho1 - rc).abs().sum()) / prob.nelec)
+            for key, r in (("sad", r0), ("sad_weighted", r0w), ("learned", r1)):
+                its[key].append(r.iterations)
+                e[key].append(0.5 * r.energies["total_ry"] / prob.n_atoms)
+            natoms = prob.n_atoms
+            prob.close()
+        de = max(abs(x - e["sad"][0]) for k in e for x in e[k])
+        rows.append({"kind": kind, "atoms": natoms, "points": dom["nx"] ** 3, "l1": l1, "its": its, "de": de})
+        print(f"[learned] held-out {g}: {kind} ({natoms} atoms) L1 SAD {l1[0]:.3f} learned {l1[1]:.4f}; its "
+              f"SAD {its['sad']} SAD-weighted {its['sad_weighted']} learned {its['learned']}", flush=True)
+    mean = {k: float(np.mean([np.mean(r["its"][k]) for r in rows])) for k in ("sad", "sad_weighted", "learned")}
+    lines = ["# Learned start: held-out evaluation", "",
+             f"Checkpoint `{os.path.basename(a.checkpoint or default_checkpoint())}`; {a.geometries} geometries drawn "
+             f"with seed {a.heldout_seed} (distorted alkanes CnH2n+2 and H-passivated Si clusters, the training kinds, "
+             f"seeds the training never used), cube {TRAIN_RADIUS_ANG} A radius at h = {TRAIN_H_ANG} A (64^3); "
+             f"mixed precision, stop at |dE| < 1e-6 Ha/atom; start-vector seeds 1..{a.seeds}. "
+             f"Command: `python -m parsec_python_amd.learned evaluate --geometries {a.geometries} --seeds {a.seeds}`. "
+             f"Wall time {time.perf_counter() - t0:.0f} s.", "",
+             "| # | kind | atoms | start L1 / e- SAD | learned | SCF its SAD | SAD + SAD-weighted vectors | learned "
+             "(+ learned-weighted vectors) | max energy spread (Ha/atom) |",
+             "|---|---|---|---|---|---|---|---|---|"]
+    for i, r in enumerate(rows):
+        lines.append(f"| {i} | {r['kind']} | {r['atoms']} | {r['l1'][0]:.3f} | {r['l1'][1]:.4f} | "
+                     f"{' '.join(map(str, r['its']['sad']))} | {' '.join(map(str, r['its']['sad_weighted']))} | "
+                     f"{' '.join(map(str, r['its']['learned']))} | {r['de']:.1e} |")
+    lines += ["", f"Mean SCF iterations: SAD {mean['sad']:.2f}, SAD with SAD-weighted start vectors "
+                  f"{mean['sad_weighted']:.2f}, learned start {mean['learned']:.2f}.", ""]
+    with open(a.out, "w") as f:
+        f.write("\n".join(lines))
+    print(mean, flush=True)
+    return rows, mean
 
 
 def train_main(argv=None):
@@ -513,4 +589,8 @@ def train_main(argv=None):
 
 
 if __name__ == "__main__":
-    train_main()
+    import sys
+    if len(sys.argv) > 1 and sys.argv[1] == "evaluate":
+        evaluate_main(sys.argv[2:])
+    else:
+        train_main()
