@@ -95,11 +95,20 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
                 double* lam_host);
 int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
 /* eigensolver used by pc_dense_rr: 0 = rocSOLVER divide and conquer (syevd),
- * 1 = Jacobi (syevj), 2 = one-launch Householder tridiagonalization + rocSOLVER
- * stedc + ormtr (default; 32 workgroups exchanging through agent-scope atomics
- * for 128 <= k <= 320, one workgroup up to 512, larger k uses syevd),
- * 3 = as 2 with the one-workgroup reduction always */
+ * 1 = Jacobi (syevj), 2 = one-launch Householder tridiagonalization + the
+ * tridiagonal eigensolver of this library (multisection bisection, inverse
+ * iteration, Gram-Schmidt and Rayleigh-Ritz within clusters of close
+ * eigenvalues; rocSOLVER stedc for a cluster of more than 32) + ormtr
+ * (default; 32 workgroups exchanging through agent-scope atomics for
+ * 128 <= k <= 320, one workgroup up to 512, larger k uses syevd), 3 = as 2
+ * with the one-workgroup reduction always, 4 = as 2 with stedc always */
 int pc_dense_set_method(void* ctx, int method);
+/* how pc_dense_rr applies R^-1: 0 = trtri + trmm (default), 1 = trsm in
+ * place (no counterpart in the reference: its QR gives W directly) */
+int pc_dense_set_triangular(void* ctx, int mode);
+/* steps repeated with stedc because a cluster was too large (no counterpart
+ * in the reference; for tests) */
+int pc_dense_tri_fallbacks(void* ctx, int* out);
 /* test control of the multi-workgroup reduction: spins (> 0) = polls of one
  * wait before it gives up and pc_dense_rr repeats the reduction with one
  * workgroup; reset = 1 re-enables it; out_aborts = reductions repeated so far.

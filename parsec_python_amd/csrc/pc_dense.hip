@@ -15,6 +15,21 @@
 
 namespace {
 
+constexpr int kTriMaxM = 32;
+constexpr int kTriClusterFlag = -7;
+constexpr int kTriMaxK = 512;
+
+struct TriWs {
+  double* u1;   // [k*k] U diagonal
+  double* u2;   // [k*k] U first superdiagonal
+  double* u3;   // [k*k] U second superdiagonal (fill-in of pivoting)
+  double* l;    // [k*k] multipliers
+  double* x;    // [k*k] eigenvector iterates
+  unsigned char* piv;   // [k*k]
+  double* lam;  // [k] bisection eigenvalues
+  double* par;  // [4] gl, gu, ||T||, pivmin
+};
+
 struct DenseCtx {
   rocblas_handle h;
   hipStream_t stream;
@@ -24,7 +39,7 @@ struct DenseCtx {
   double* out;      // kmax + 4 : eigenvalues then 3 info codes (as doubles)
   int* info;        // 4
   double* host_out; // pinned, kmax + 4
-  int method;       // 0: syevd, 1: Jacobi (syevj), 2: one-launch tridiagonalization + stedc + ormtr
+  int method;       // 0: syevd, 1: Jacobi (syevj), 2: one-launch tridiagonalization + tri_* + ormtr
   double* tau;      // kmax (method 2: reflector scalars)
   double* P;        // method 2: padded copy of G, 64*ceil(kmax/64) x (kmax+1)
   double* resid;    // syevj: off-diagonal norm at exit
@@ -37,6 +52,11 @@ struct DenseCtx {
   double* S0;            // kmax*kmax: copy of the overlap matrix (shifted retry)
   double* G0;            // kmax*kmax: copy of the projected matrix (shifted retry)
   int shifted;           // Rayleigh-Ritz steps that needed a shifted Cholesky factor
+  TriWs tw;              // tridiagonal eigensolver workspace (kmax <= kTriMaxK)
+  double* tri_lam;       // kmax: its eigenvalues before they replace the diagonal
+  bool tri_ok;           // false after a cluster exceeded kTriMaxM (stedc from then on)
+  bool trsm;             // R^-1 applied by triangular solves (else trtri + trmm)
+  int tri_fallbacks;     // steps repeated with stedc
 };
 
 // S += shift * max_i S_ii on the diagonal (one workgroup of 256 threads)
@@ -502,6 +522,353 @@ __global__ __launch_bounds__(256) void sytrd_lower_mwg(const double* __restrict_
   }
 }
 
+// ---- eigenpairs of the symmetric tridiagonal matrix (d, e) -----------------
+// Replaces rocSOLVER's stedc (five merge levels, about a hundred launches at
+// k = 275) with three launches:
+//   tri_bisect:  eigenvalue i by multisection on the Sturm count, one wavefront
+//                per eigenvalue: each round its 64 lanes count the eigenvalues
+//                below 64 equally spaced shifts of the current interval and the
+//                interval shrinks 65-fold, until it is 2 eps |lambda| +
+//                1e-2 eps ||T|| wide (about 10 rounds; two interleaved chains
+//                of 128 shifts per round measured slower: the step is bound
+//                by FP64 issue, not latency);
+//   tri_invit:   inverse iteration, one thread per eigenvalue: LU with partial
+//                pivoting of T - sigma I, three solves from a fixed start
+//                vector, each normalised; equal shifts are separated by
+//                10 eps max(|lambda|, 1e-3 ||T||) per repeat;
+//   tri_cluster: eigenvalues closer than 1e-3 ||T|| form a cluster (one
+//                workgroup each): its vectors are orthonormalised (modified
+//                Gram-Schmidt, twice) and rotated to the eigenvectors of their
+//                projected m x m matrix (Jacobi), so vectors of close or equal
+//                eigenvalues come out orthogonal and exact within the cluster.
+// Per-thread arrays live in a workspace interleaved as [j * k + i] (element j
+// of eigenvalue i), so the k threads' accesses are contiguous.  A cluster of
+// more than kTriMaxM eigenvalues is reported in info (kTriClusterFlag) and the
+// caller repeats the step with rocSOLVER's stedc.
+// 1/q to about one ulp: the hardware reciprocal and two Newton steps (a
+// shorter dependent chain than IEEE division; q is never 0 where it is used)
+__device__ __forceinline__ double tri_rcp(double q) {
+  double r = __builtin_amdgcn_rcp(q);
+  double t = fma(-q, r, 1.0);
+  r = fma(r, t, r);
+  t = fma(-q, r, 1.0);
+  return fma(r, t, r);
+}
+
+__device__ __forceinline__ void tri_bounds(const double* d, const double* e, int k, double* red, double* out4) {
+  // Gershgorin interval, ||T|| and the pivot threshold (256 threads)
+  double gl = 1e300, gu = -1e300, em = 0.0;
+  for (int j = threadIdx.x; j < k; j += 256) {
+    const double a = j > 0 ? fabs(e[j - 1]) : 0.0, b = j + 1 < k ? fabs(e[j]) : 0.0;
+    gl = fmin(gl, d[j] - a - b);
+    gu = fmax(gu, d[j] + a + b);
+    if (j + 1 < k) em = fmax(em, e[j] * e[j]);
+  }
+  red[threadIdx.x] = gl;
+  red[256 + threadIdx.x] = gu;
+  red[512 + threadIdx.x] = em;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      red[threadIdx.x] = fmin(red[threadIdx.x], red[threadIdx.x + o]);
+      red[256 + threadIdx.x] = fmax(red[256 + threadIdx.x], red[256 + threadIdx.x + o]);
+      red[512 + threadIdx.x] = fmax(red[512 + threadIdx.x], red[512 + threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  const double eps = 2.220446049250313e-16;
+  const double tn = fmax(fmax(fabs(red[0]), fabs(red[256])), 1e-300);
+  out4[0] = red[0] - 2.0 * eps * tn * k;
+  out4[1] = red[256] + 2.0 * eps * tn * k;
+  out4[2] = tn;
+  out4[3] = 2.2250738585072014e-308 * fmax(1.0, red[512]);
+}
+
+__global__ __launch_bounds__(256) void tri_bisect(const double* __restrict__ d, const double* __restrict__ e, int k,
+                                                  TriWs ws) {
+  __shared__ double sd[kTriMaxK], se2[kTriMaxK], red[768];
+  __shared__ double par[4];
+  for (int j = threadIdx.x; j < k; j += 256) {
+    sd[j] = d[j];
+    se2[j] = j + 1 < k ? e[j] * e[j] : 0.0;
+  }
+  double b4[4];
+  tri_bounds(d, e, k, red, b4);
+  if (threadIdx.x == 0) {
+    for (int t = 0; t < 4; ++t) par[t] = b4[t];
+    if (blockIdx.x == 0)
+      for (int t = 0; t < 4; ++t) ws.par[t] = b4[t];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (i >= k) return;
+  const double pivmin = par[3];
+  double lo = par[0], hi = par[1];
+  for (int rnd = 0; rnd < 16; ++rnd) {
+    const double w = hi - lo;
+    const double x = lo + w * ((double)(lane + 1) / 65.0);
+    double q = sd[0] - x;
+    int cnt = q < 0.0;
+    for (int j = 1; j < k; ++j) {
+      if (fabs(q) < pivmin) q = -pivmin;
+      q = sd[j] - x - se2[j - 1] * tri_rcp(q);
+      cnt += q < 0.0;
+    }
+    // counts grow with the shift: the first lane counting more than i
+    const unsigned long long b = __ballot(cnt > i);
+    const int f = b ? __ffsll((long long)b) - 1 : 64;
+    const double nlo = f == 0 ? lo : lo + w * ((double)f / 65.0);
+    const double nhi = f == 64 ? hi : lo + w * ((double)(f + 1) / 65.0);
+    lo = nlo;
+    hi = nhi;
+    if (hi - lo <= 4.440892098500626e-16 * fmax(fabs(lo), fabs(hi)) + 2.220446049250313e-18 * par[2] + pivmin)
+      break;
+  }
+  if (lane == 0) ws.lam[i] = 0.5 * (lo + hi);
+}
+
+// 32 threads per workgroup, each workgroup on its own CU: the iterate x of
+// each thread sits in LDS ([j][thread], 8 B x 32 per row, up to 128 KB) and
+// the factors in the interleaved workspace, read in blocks of TB rows whose
+// loads go out before the block's dependent chain.
+constexpr int kInvT = 32;
+__global__ __launch_bounds__(kInvT) void tri_invit(const double* __restrict__ d, const double* __restrict__ e, int k,
+                                                   TriWs ws) {
+  __shared__ double sdd[kTriMaxK], sde[kTriMaxK];
+  __shared__ double sx[kTriMaxK * kInvT];
+  for (int j = threadIdx.x; j < k; j += kInvT) {
+    sdd[j] = d[j];
+    sde[j] = j + 1 < k ? e[j] : 0.0;
+  }
+  __syncthreads();
+  const int lt = threadIdx.x;
+  const int i = blockIdx.x * kInvT + lt;
+  if (i >= k) return;
+  const double eps = 2.220446049250313e-16;
+  const double tn = ws.par[2];
+  const double tiny = eps * tn;
+  // separate equal shifts: the r-th repeat of a value moves up by r deltas
+  const double delta = 10.0 * eps * fmax(fabs(ws.lam[i]), 1e-3 * tn);
+  int rep = 0;
+  for (int t = i; t > 0 && ws.lam[t] - ws.lam[t - 1] < delta; --t) ++rep;
+  const double sg = ws.lam[i] + rep * delta;
+  auto at = [&](int j) { return (long long)j * k + i; };
+  auto X = [&](int j) -> double& { return sx[j * kInvT + lt]; };
+  // LU with partial pivoting of T - sg I; row j of U: u1 (j,j), u2 (j,j+1), u3 (j,j+2)
+  double cd = sdd[0] - sg, cu = k > 1 ? sde[0] : 0.0;
+  for (int j = 0; j + 1 < k; ++j) {
+    const double sub = sde[j], nd = sdd[j + 1] - sg, nu = j + 2 < k ? sde[j + 1] : 0.0;
+    if (fabs(cd) >= fabs(sub)) {
+      const double m = cd != 0.0 ? sub * tri_rcp(cd) : 0.0;
+      ws.u1[at(j)] = cd; ws.u2[at(j)] = cu; ws.u3[at(j)] = 0.0; ws.l[at(j)] = m; ws.piv[at(j)] = 0;
+      cd = nd - m * cu;
+      cu = nu;
+    } else {
+      const double m = cd * tri_rcp(sub);
+      ws.u1[at(j)] = sub; ws.u2[at(j)] = nd; ws.u3[at(j)] = nu; ws.l[at(j)] = m; ws.piv[at(j)] = 1;
+      cd = cu - m * nd;
+      cu = -m * nu;
+    }
+  }
+  ws.u1[at(k - 1)] = cd;
+  // fixed pseudo-random start vector in [-0.5, 0.5) (no two eigenvalues start alike)
+  for (int j = 0; j < k; ++j) {
+    unsigned h = (unsigned)j * 0x9E3779B1u ^ (unsigned)(i + 1) * 0x85EBCA77u;
+    h ^= h >> 15;
+    h *= 0x2C1B3C6Du;
+    h ^= h >> 12;
+    X(j) = (double)(h & 0xFFFFFFu) * (1.0 / 16777216.0) - 0.5;
+  }
+  constexpr int TB = 16;
+  double sc = 1.0;
+  for (int it = 0; it < 3; ++it) {
+    double yc = X(0);
+    for (int j0 = 0; j0 + 1 < k; j0 += TB) {
+      double lb[TB];
+      unsigned char pb[TB];
+#pragma unroll
+      for (int t = 0; t < TB; ++t) {
+        const int j = j0 + t;
+        if (j + 1 < k) {
+          lb[t] = ws.l[at(j)];
+          pb[t] = ws.piv[at(j)];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TB; ++t) {
+        const int j = j0 + t;
+        if (j + 1 < k) {
+          const double yn = X(j + 1);
+          if (pb[t]) {
+            X(j) = yn;
+            yc = yc - lb[t] * yn;
+          } else {
+            X(j) = yc;
+            yc = yn - lb[t] * yc;
+          }
+        }
+      }
+    }
+    X(k - 1) = yc;
+    double y1 = 0.0, y2 = 0.0, nrm = 0.0;
+    for (int j1 = k - 1; j1 >= 0; j1 -= TB) {
+      double ub1[TB], ub2[TB], ub3[TB];
+#pragma unroll
+      for (int t = 0; t < TB; ++t) {
+        const int j = j1 - t;
+        if (j >= 0) {
+          ub1[t] = ws.u1[at(j)];
+          ub2[t] = ws.u2[at(j)];
+          ub3[t] = ws.u3[at(j)];
+        }
+      }
+#pragma unroll
+      for (int t = 0; t < TB; ++t) {
+        const int j = j1 - t;
+        if (j >= 0) {
+          const double u = ub1[t];
+          const double p = fabs(u) > tiny ? u : (u >= 0.0 ? tiny : -tiny);
+          // the previous pass's normalisation is folded into this one
+          const double y = (sc * X(j) - ub2[t] * y1 - ub3[t] * y2) * tri_rcp(p);
+          X(j) = y;
+          y2 = y1;
+          y1 = y;
+          nrm += y * y;
+        }
+      }
+    }
+    sc = 1.0 / sqrt(nrm);
+  }
+  for (int j = 0; j < k; ++j) ws.x[at(j)] = sc * X(j);
+}
+
+__device__ __forceinline__ double tri_block_sum(double v, double* red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// Z (k x k, ld k) and the eigenvalues (ascending) into lam_out; info_flag
+// receives kTriClusterFlag when a cluster is larger than kTriMaxM.
+__global__ __launch_bounds__(256) void tri_cluster(const double* __restrict__ d, const double* __restrict__ e, int k,
+                                                   TriWs ws, double* Z, double* lam_out, int* info_flag) {
+  __shared__ double red[256];
+  __shared__ double C[kTriMaxM * kTriMaxM], U[kTriMaxM * kTriMaxM], wv[kTriMaxM];
+  __shared__ int order[kTriMaxM];
+  const int i = blockIdx.x;
+  const double ortol = 1e-3 * ws.par[2];
+  if (i > 0 && ws.lam[i] - ws.lam[i - 1] < ortol) return;   // not the first of its cluster
+  int j = i;
+  while (j + 1 < k && ws.lam[j + 1] - ws.lam[j] < ortol) ++j;
+  const int m = j - i + 1;
+  auto X = [&](int r, int c) -> double& { return ws.x[(long long)r * k + i + c]; };
+  if (m == 1) {
+    for (int r = threadIdx.x; r < k; r += 256) Z[(long long)i * k + r] = X(r, 0);
+    if (threadIdx.x == 0) lam_out[i] = ws.lam[i];
+    return;
+  }
+  if (m > kTriMaxM) {
+    if (threadIdx.x == 0) *info_flag = kTriClusterFlag;
+    return;
+  }
+  // modified Gram-Schmidt, twice
+  for (int pass = 0; pass < 2; ++pass) {
+    for (int a = 0; a < m; ++a) {
+      for (int b = 0; b < a; ++b) {
+        double s = 0.0;
+        for (int r = threadIdx.x; r < k; r += 256) s += X(r, b) * X(r, a);
+        s = tri_block_sum(s, red);
+        for (int r = threadIdx.x; r < k; r += 256) X(r, a) -= s * X(r, b);
+        __syncthreads();
+      }
+      double s = 0.0;
+      for (int r = threadIdx.x; r < k; r += 256) s += X(r, a) * X(r, a);
+      s = tri_block_sum(s, red);
+      const double inv = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
+      for (int r = threadIdx.x; r < k; r += 256) X(r, a) *= inv;
+      __syncthreads();
+    }
+  }
+  // C = Zc^T T Zc
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b <= a; ++b) {
+      double s = 0.0;
+      for (int r = threadIdx.x; r < k; r += 256) {
+        double tz = d[r] * X(r, b);
+        if (r > 0) tz += e[r - 1] * X(r - 1, b);
+        if (r + 1 < k) tz += e[r] * X(r + 1, b);
+        s += X(r, a) * tz;
+      }
+      s = tri_block_sum(s, red);
+      if (threadIdx.x == 0) {
+        C[a * kTriMaxM + b] = s;
+        C[b * kTriMaxM + a] = s;
+      }
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    // cyclic Jacobi on the m x m matrix (nearly diagonal: few sweeps)
+    for (int a = 0; a < m; ++a)
+      for (int b = 0; b < m; ++b) U[a * kTriMaxM + b] = a == b ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+      double off = 0.0, dia = 0.0;
+      for (int a = 0; a < m; ++a) {
+        dia += C[a * kTriMaxM + a] * C[a * kTriMaxM + a];
+        for (int b = a + 1; b < m; ++b) off += C[a * kTriMaxM + b] * C[a * kTriMaxM + b];
+      }
+      if (off <= 1e-34 * dia || off == 0.0) break;
+      for (int p = 0; p < m - 1; ++p)
+        for (int q = p + 1; q < m; ++q) {
+          const double apq = C[p * kTriMaxM + q];
+          if (apq == 0.0) continue;
+          const double app = C[p * kTriMaxM + p], aqq = C[q * kTriMaxM + q];
+          const double th = (aqq - app) / (2.0 * apq);
+          const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+          const double cs = 1.0 / sqrt(t * t + 1.0), sn = t * cs;
+          for (int r = 0; r < m; ++r) {   // C <- J^T C J, U <- U J
+            const double crp = C[r * kTriMaxM + p], crq = C[r * kTriMaxM + q];
+            C[r * kTriMaxM + p] = cs * crp - sn * crq;
+            C[r * kTriMaxM + q] = sn * crp + cs * crq;
+          }
+          for (int r = 0; r < m; ++r) {
+            const double cpr = C[p * kTriMaxM + r], cqr = C[q * kTriMaxM + r];
+            C[p * kTriMaxM + r] = cs * cpr - sn * cqr;
+            C[q * kTriMaxM + r] = sn * cpr + cs * cqr;
+          }
+          for (int r = 0; r < m; ++r) {
+            const double urp = U[r * kTriMaxM + p], urq = U[r * kTriMaxM + q];
+            U[r * kTriMaxM + p] = cs * urp - sn * urq;
+            U[r * kTriMaxM + q] = sn * urp + cs * urq;
+          }
+        }
+    }
+    // ascending order of the cluster's eigenvalues
+    for (int a = 0; a < m; ++a) { wv[a] = C[a * kTriMaxM + a]; order[a] = a; }
+    for (int a = 1; a < m; ++a) {
+      const int o = order[a];
+      int b = a - 1;
+      while (b >= 0 && wv[order[b]] > wv[o]) { order[b + 1] = order[b]; --b; }
+      order[b + 1] = o;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < m) lam_out[i + threadIdx.x] = wv[order[threadIdx.x]];
+  for (int r = threadIdx.x; r < k; r += 256)
+    for (int a = 0; a < m; ++a) {
+      const int c = order[a];
+      double s = 0.0;
+      for (int b = 0; b < m; ++b) s += X(r, b) * U[b * kTriMaxM + c];
+      Z[(long long)(i + a) * k + r] = s;
+    }
+}
+
 #define PC_RB(call)                                                                          \
   do {                                                                                       \
     rocblas_status s_ = (call);                                                              \
@@ -540,6 +907,23 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipHostMalloc(&c->host_out, (size_t)(kmax + 4) * sizeof(double), hipHostMallocDefault));
   PC_HIP(hipMalloc(&c->sync, kTrdSyncInts * sizeof(int)));
   PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
+  c->trsm = false;
+  c->tri_ok = kmax <= kTriMaxK;
+  c->tri_fallbacks = 0;
+  c->tri_lam = nullptr;
+  c->tw = TriWs{};
+  if (c->tri_ok) {
+    const size_t kk = (size_t)kmax * kmax;
+    PC_HIP(hipMalloc(&c->tw.u1, 5 * kk * sizeof(double)));
+    c->tw.u2 = c->tw.u1 + kk;
+    c->tw.u3 = c->tw.u2 + kk;
+    c->tw.l = c->tw.u3 + kk;
+    c->tw.x = c->tw.l + kk;
+    PC_HIP(hipMalloc(&c->tw.piv, kk));
+    PC_HIP(hipMalloc(&c->tw.lam, (2 * (size_t)kmax + 4) * sizeof(double)));
+    c->tri_lam = c->tw.lam + kmax;
+    c->tw.par = c->tri_lam + kmax;
+  }
   PC_HIP(hipMalloc(&c->S0, (size_t)kmax * kmax * sizeof(double)));
   PC_HIP(hipMalloc(&c->G0, (size_t)kmax * kmax * sizeof(double)));
   c->shifted = 0;
@@ -560,6 +944,7 @@ int pc_dense_destroy(void* ctx) {
   (void)hipFree(c->tmp); (void)hipFree(c->E); (void)hipFree(c->out); (void)hipFree(c->info);
   (void)hipFree(c->resid); (void)hipFree(c->sweeps); (void)hipFree(c->tau); (void)hipFree(c->P);
   (void)hipFree(c->S0); (void)hipFree(c->G0);
+  if (c->tw.u1) { (void)hipFree(c->tw.u1); (void)hipFree(c->tw.piv); (void)hipFree(c->tw.lam); }
   (void)hipHostFree(c->host_out);
   delete c;
   return pc::PC_OK;
@@ -600,23 +985,32 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   hipLaunchKernelGGL(zero_info, dim3(1), dim3(64), 0, c->stream, c->info);
   PC_LAUNCH_CHECK();
   PC_RB(rocsolver_dpotrf(c->h, rocblas_fill_upper, k, S, lds, c->info));
-  PC_RB(rocsolver_dtrtri(c->h, rocblas_fill_upper, rocblas_diagonal_non_unit, k, S, lds, c->info + 1));
-  // tmp = Gy R^-1
-  PC_RB(rocblas_dtrmm(c->h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                      rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg, c->tmp, k));
-  // Gy = R^-T tmp
-  PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
-                      rocblas_diagonal_non_unit, k, k, &one, S, lds, c->tmp, k, Gy, ldg));
+  if (c->trsm) {
+    // Gy <- R^-T Gy R^-1 by two triangular solves in place
+    PC_RB(rocblas_dtrsm(c->h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg));
+    PC_RB(rocblas_dtrsm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                        rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg));
+  } else {
+    PC_RB(rocsolver_dtrtri(c->h, rocblas_fill_upper, rocblas_diagonal_non_unit, k, S, lds, c->info + 1));
+    // tmp = Gy R^-1
+    PC_RB(rocblas_dtrmm(c->h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                        rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg, c->tmp, k));
+    // Gy = R^-T tmp
+    PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
+                        rocblas_diagonal_non_unit, k, k, &one, S, lds, c->tmp, k, Gy, ldg));
+  }
   hipLaunchKernelGGL(symmetrize, dim3((k * k + 255) / 256), dim3(256), 0, c->stream, Gy, k, ldg);
   PC_LAUNCH_CHECK();
   for (int attempt = 0;; ++attempt) {
     const double* Q = Gy;   // eigenvectors of G
     int ldq = ldg;
     bool mwg = false;
-    if ((c->method == 2 || c->method == 3) && c->P && k <= kTrdMax) {
-      // G = H T H^T, T = Z diag(lam) Z^T (stedc), Q = H Z (ormtr)
+    bool tri = false;
+    if ((c->method == 2 || c->method == 3 || c->method == 4) && c->P && k <= kTrdMax) {
+      // G = H T H^T, T = Z diag(lam) Z^T (tri_* kernels or stedc), Q = H Z (ormtr)
       const int nch = (k + 63) / 64;
-      mwg = c->method == 2 && c->mwg_ok && k >= kTrdMwgMin && k <= kTrdMwgMax;
+      mwg = (c->method == 2 || c->method == 4) && c->mwg_ok && k >= kTrdMwgMin && k <= kTrdMwgMax;
       if (mwg) {
         PC_HIP(hipMemsetAsync(c->sync, 0, kTrdSyncInts * sizeof(int), c->stream));
         switch (nch) {
@@ -641,7 +1035,20 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
         }
       }
       PC_LAUNCH_CHECK();
-      PC_RB(rocsolver_dstedc(c->h, rocblas_evect_tridiagonal, k, c->out, c->E, c->tmp, k, c->info + 2));
+      tri = c->method != 4 && c->tri_ok && k <= kTriMaxK;
+      if (tri) {
+        hipLaunchKernelGGL(tri_bisect, dim3((k + 3) / 4), dim3(256), 0, c->stream, c->out, c->E, k, c->tw);
+        PC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(tri_invit, dim3((k + kInvT - 1) / kInvT), dim3(kInvT), 0, c->stream, c->out, c->E, k,
+                           c->tw);
+        PC_LAUNCH_CHECK();
+        hipLaunchKernelGGL(tri_cluster, dim3(k), dim3(256), 0, c->stream, c->out, c->E, k, c->tw, c->tmp,
+                           c->tri_lam, c->info + 2);
+        PC_LAUNCH_CHECK();
+        PC_HIP(hipMemcpyAsync(c->out, c->tri_lam, k * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+      } else {
+        PC_RB(rocsolver_dstedc(c->h, rocblas_evect_tridiagonal, k, c->out, c->E, c->tmp, k, c->info + 2));
+      }
       PC_RB(rocsolver_dormtr(c->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, k, k, c->P,
                              64 * nch, c->tau, c->tmp, k));
       Q = c->tmp;
@@ -654,19 +1061,37 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
                              c->info + 2));
     }
     // T = R^-1 Q
-    PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
-                        rocblas_diagonal_non_unit, k, k, &one, S, lds, Q, ldq, T, ldt));
+    if (c->trsm) {
+      PC_HIP(hipMemcpy2DAsync(T, ldt * sizeof(double), Q, ldq * sizeof(double), k * sizeof(double), k,
+                              hipMemcpyDeviceToDevice, c->stream));
+      PC_RB(rocblas_dtrsm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, k, k, &one, S, lds, T, ldt));
+    } else {
+      PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
+                          rocblas_diagonal_non_unit, k, k, &one, S, lds, Q, ldq, T, ldt));
+    }
     PC_HIP(hipMemcpyAsync(lam_dev, c->out, k * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     hipLaunchKernelGGL(pack_out, dim3(1), dim3(64), 0, c->stream, c->info, c->out, k);
     PC_LAUNCH_CHECK();
     PC_HIP(hipMemcpyAsync(c->host_out, c->out, (k + 4) * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     PC_HIP(hipStreamSynchronize(c->stream));
-    if (mwg && c->host_out[k + 3] != 0.0 && attempt == 0) {
+    // (a multi-workgroup reduction that gave up leaves a meaningless
+    // tridiagonal matrix: that is repeated first, whatever came after it)
+    if (tri && !(mwg && c->host_out[k + 3] != 0.0) && (int)c->host_out[k + 2] == kTriClusterFlag &&
+        attempt < 2) {
+      // a cluster of close eigenvalues larger than the kernels take: repeat
+      // the step with rocSOLVER's stedc (G is unchanged) and keep using it
+      c->tri_ok = false;
+      c->tri_fallbacks += 1;
+      PC_HIP(hipMemsetAsync(c->info + 2, 0, sizeof(int), c->stream));
+      continue;
+    }
+    if (mwg && c->host_out[k + 3] != 0.0 && attempt < 2) {
       // a workgroup of the multi-workgroup reduction gave up waiting: repeat
       // with the one-workgroup kernel (G is unchanged) and keep using it
       c->mwg_ok = false;
       c->mwg_aborts += 1;
-      PC_HIP(hipMemsetAsync(c->info + 3, 0, sizeof(int), c->stream));
+      PC_HIP(hipMemsetAsync(c->info + 2, 0, 2 * sizeof(int), c->stream));
       continue;
     }
     break;
@@ -695,12 +1120,15 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
 // (rocSOLVER syevd), 1 = cyclic Jacobi (rocSOLVER syevj, tolerance = machine
 // precision, at most 100 sweeps), 2 = Householder tridiagonalization in one
 // launch (32 workgroups for 128 <= k <= 320, one workgroup otherwise; k > 512
-// uses syevd) + rocSOLVER stedc + ormtr, 3 = as 2 with one workgroup always.
+// uses syevd) + the tri_* tridiagonal eigensolver (rocSOLVER stedc if a
+// cluster of close eigenvalues exceeds kTriMaxM) + ormtr, 3 = as 2 with one
+// workgroup always, 4 = as 2 with stedc always.
 int pc_dense_set_method(void* ctx, int method) {
   DenseCtx* c = static_cast<DenseCtx*>(ctx);
   PC_CHECK_ARG(c, "null context");
-  PC_CHECK_ARG(method >= 0 && method <= 3,
-               "method must be 0 (syevd), 1 (syevj), 2 (tridiagonalization + stedc) or 3 (2, one workgroup)");
+  PC_CHECK_ARG(method >= 0 && method <= 4,
+               "method must be 0 (syevd), 1 (syevj), 2 (tridiagonalization + tridiagonal eigensolver), "
+               "3 (2, one workgroup) or 4 (tridiagonalization + stedc)");
   c->method = method;
   return pc::PC_OK;
 }
@@ -719,6 +1147,25 @@ int pc_dense_trd_control(void* ctx, int spins, int reset, int* out_aborts) {
     c->mwg_ok = true;
     c->mwg_aborts = 0;
   }
+  return pc::PC_OK;
+}
+
+// How R^-1 is applied in pc_dense_rr: 0 = explicit inverse (trtri, then
+// three trmm), 1 = triangular solves in place (three trsm).
+int pc_dense_set_triangular(void* ctx, int mode) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c, "null context");
+  PC_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (trtri + trmm) or 1 (trsm)");
+  c->trsm = mode == 1;
+  return pc::PC_OK;
+}
+
+// Steps of this context whose tridiagonal eigenproblem went to stedc because
+// a cluster was too large for the tri_* kernels.
+int pc_dense_tri_fallbacks(void* ctx, int* out) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c && out, "null pointer");
+  *out = c->tri_fallbacks;
   return pc::PC_OK;
 }
 

@@ -205,15 +205,19 @@ def test_dense_rr_generalized_eig(gpu):
     la.close()
 
 
-@pytest.mark.parametrize("method", [0, 1, 2, 3])
+@pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
 @pytest.mark.parametrize("k,kind", [(1, "rand"), (2, "rand"), (3, "rand"), (65, "rand"), (128, "rand"),
-                                    (275, "degenerate"), (300, "rand"), (320, "degenerate"), (520, "rand")])
+                                    (275, "degenerate"), (275, "band"), (300, "rand"), (320, "degenerate"),
+                                    (520, "rand")])
 def test_dense_rr_methods(gpu, method, k, kind):
     """Every eigensolver of the dense stage (syevd, syevj, tridiagonalization
-    + stedc with 32 workgroups (method 2, 128 <= k <= 320) or one (method 3))
-    returns the generalized eigenpairs of (G, S): T^T S T = I, T^T G T =
-    diag(lam), lam = SciPy's.  'degenerate' has exact triple eigenvalues, as
-    the symmetric clusters give."""
+    with 32 workgroups (methods 2 and 4, 128 <= k <= 320) or one (method 3),
+    then this library's tridiagonal eigensolver (2, 3) or rocSOLVER's stedc
+    (4)) returns the generalized eigenpairs of (G, S): T^T S T = I,
+    T^T G T = diag(lam), lam = SciPy's.  'degenerate' has exact triple
+    eigenvalues, as the symmetric clusters give; 'band' packs 3/4 of the
+    spectrum into a narrow window (many close pairs), as the occupied Ritz
+    values of a large cluster do."""
     import torch
     from scipy.linalg import eigh
     from parsec_python_amd import _lib
@@ -224,9 +228,12 @@ def test_dense_rr_methods(gpu, method, k, kind):
     rng = np.random.default_rng(k + 7 * method)
     B = rng.standard_normal((max(2 * k, 4), k))
     S = B.T @ B / B.shape[0] + 0.5 * np.eye(k)
-    if kind == "degenerate":
+    if kind in ("degenerate", "band"):
         Qr, _ = np.linalg.qr(rng.standard_normal((k, k)))
-        lam0 = np.repeat(rng.uniform(-1.0, 2.0, (k + 2) // 3), 3)[:k]
+        if kind == "degenerate":
+            lam0 = np.repeat(rng.uniform(-1.0, 2.0, (k + 2) // 3), 3)[:k]
+        else:
+            lam0 = np.sort(np.concatenate([rng.uniform(-0.5, -0.3, 3 * k // 4), rng.uniform(0.0, 0.3, k - 3 * k // 4)]))
         A = (Qr * lam0) @ Qr.T
         L = np.linalg.cholesky(S)
         G = L @ A @ L.T                      # (G, S) has the spectrum lam0
@@ -235,15 +242,18 @@ def test_dense_rr_methods(gpu, method, k, kind):
         G = M + M.T
     la = BlockLinalg(10, k)
     _lib.call("pc_dense_set_method", la.dense, method)
-    Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
-    Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
-    T, lam = la.dense_rr(Sd, Gd, k)
     ref = eigh(G, S, eigvals_only=True)
     scale = max(1.0, np.abs(ref).max())
-    assert np.allclose(lam, ref, rtol=0, atol=1e-11 * scale * k)
-    Tm = T.cpu().numpy()[:k * k].reshape(k, k).T  # column-major
-    assert np.abs(Tm.T @ S @ Tm - np.eye(k)).max() < 1e-10 * k
-    assert np.abs(Tm.T @ G @ Tm - np.diag(lam)).max() < 1e-10 * scale * k
+    # R^-1 as trtri + trmm, and (method 2) also as triangular solves
+    for tri_mode in ((0, 1) if method == 2 else (0,)):
+        _lib.call("pc_dense_set_triangular", la.dense, tri_mode)
+        Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
+        Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
+        T, lam = la.dense_rr(Sd, Gd, k)
+        assert np.allclose(lam, ref, rtol=0, atol=1e-11 * scale * k)
+        Tm = T.cpu().numpy()[:k * k].reshape(k, k).T  # column-major
+        assert np.abs(Tm.T @ S @ Tm - np.eye(k)).max() < 1e-10 * k
+        assert np.abs(Tm.T @ G @ Tm - np.diag(lam)).max() < 1e-10 * scale * k
     aborts = C.c_int(-1)
     _lib.call("pc_dense_trd_control", la.dense, 0, 0, C.byref(aborts))
     assert aborts.value == 0          # the multi-workgroup waits all completed
@@ -281,6 +291,76 @@ def test_dense_rr_mwg_fallback(gpu, k):
     assert aborts.value in (0, 1)     # at most once: the context then keeps one workgroup
     _lib.call("pc_dense_trd_control", la.dense, 1 << 21, 1, None)
     la.close()
+
+
+def test_dense_rr_large_cluster_falls_back_to_stedc(gpu):
+    """A cluster of 40 eigenvalues within 1e-7 of each other is more than the
+    tridiagonal eigensolver's clusters take (32): the step is repeated with
+    rocSOLVER's stedc once, the context keeps stedc, and the pairs are right."""
+    import torch
+    from scipy.linalg import eigh
+    from parsec_python_amd import _lib
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    k = 128
+    rng = np.random.default_rng(77)
+    Qr, _ = np.linalg.qr(rng.standard_normal((k, k)))
+    lam0 = np.sort(np.concatenate([0.2 + 1e-7 * rng.standard_normal(40), rng.uniform(-1.0, 1.0, k - 40)]))
+    G = (Qr * lam0) @ Qr.T
+    G = 0.5 * (G + G.T)
+    S = np.eye(k)
+    la = BlockLinalg(10, k)
+    fb = C.c_int(-1)
+    for rep in range(2):
+        T, lam = la.dense_rr(torch.tensor(S.T.copy(), device="cuda").view(-1),
+                             torch.tensor(G.T.copy(), device="cuda").view(-1), k)
+        ref = eigh(G, S, eigvals_only=True)
+        assert np.allclose(lam, ref, rtol=0, atol=1e-12 * k)
+        Tm = T.cpu().numpy()[:k * k].reshape(k, k).T
+        assert np.abs(Tm.T @ Tm - np.eye(k)).max() < 1e-10 * k
+        assert np.abs(Tm.T @ G @ Tm - np.diag(lam)).max() < 1e-10 * k
+    _lib.call("pc_dense_tri_fallbacks", la.dense, C.byref(fb))
+    print(f"  steps repeated with stedc: {fb.value}", flush=True)
+    assert fb.value == 1
+    la.close()
+
+
+def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
+    """Time of one Rayleigh-Ritz dense stage at k = 275 with this library's
+    tridiagonal eigensolver (method 2) and with rocSOLVER's stedc (method 4),
+    and with R^-1 applied by triangular solves; all give the same pairs.
+    Printed for the record; the only bar is that the library's eigensolver is
+    not slower than stedc."""
+    import time
+    import torch
+    from parsec_python_amd import _lib
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    k = 275
+    rng = np.random.default_rng(5)
+    B = rng.standard_normal((2 * k, k))
+    S = B.T @ B / B.shape[0] + 0.5 * np.eye(k)
+    M = rng.standard_normal((k, k))
+    G = M + M.T
+    Sd0 = torch.tensor(S.T.copy(), device="cuda").view(-1)
+    Gd0 = torch.tensor(G.T.copy(), device="cuda").view(-1)
+    out = {}
+    for method, tri_mode in ((2, 0), (4, 0), (2, 1)):
+        la = BlockLinalg(10, k)
+        _lib.call("pc_dense_set_method", la.dense, method)
+        _lib.call("pc_dense_set_triangular", la.dense, tri_mode)
+        ts = []
+        for rep in range(6):
+            Sd, Gd = Sd0.clone(), Gd0.clone()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            T, lam = la.dense_rr(Sd, Gd, k)
+            ts.append(time.perf_counter() - t0)
+        out[(method, tri_mode)] = (float(np.median(ts[1:])) * 1e3, lam)
+        la.close()
+    print(f"  dense stage k {k}: tridiagonal eigensolver {out[(2, 0)][0]:.2f} ms, stedc {out[(4, 0)][0]:.2f} ms; "
+          f"with triangular solves {out[(2, 1)][0]:.2f} ms", flush=True)
+    for key in ((2, 0), (2, 1)):
+        assert np.allclose(out[key][1], out[(4, 0)][1], rtol=0, atol=1e-12 * k * np.abs(out[(4, 0)][1]).max())
+    assert out[(2, 0)][0] <= 1.05 * out[(4, 0)][0]
 
 
 def test_density_xc_post_vs_oracle(gpu):
