@@ -56,6 +56,7 @@ struct StArgs {
   int nvec;
   int nblocks;   // real (unpadded) number of workgroups
   int nb8;       // ceil(nblocks/8)
+  int nzc, zc;   // z-chunks per slab and planes per chunk (lean and paired kernels)
   double w[8];   // scaled weights; w[0] already includes the factor 3
 };
 
@@ -394,7 +395,9 @@ constexpr int pc_lean_ql(int R) {
 // Workgroup: TY y-rows of one vector; wave w owns row w % TY and the CPW
 // x-chunks of 64 points starting at chunk (w / TY) * CPW, so a workgroup has
 // TY * SX / CPW waves and each wave keeps only CPW chunks of queue registers.
-template <typename T, int R, int SX, int TY, int CPW, bool HASX>
+// ZC: the slab's z-range is split into chunks of g.zc planes (few vectors);
+// a separate instantiation, so the unchunked kernel keeps its registers.
+template <typename T, int R, int SX, int TY, int CPW, bool HASX, bool ZC>
 __global__ __launch_bounds__(64 * TY * SX / CPW)
 #if PC_LEAN_WAVES > 0
 __attribute__((amdgpu_waves_per_eu(PC_LEAN_WAVES, PC_LEAN_WAVES)))
@@ -419,8 +422,11 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
   const int bid = blockIdx.x;
   const int L = (bid & 7) * g.nb8 + (bid >> 3);
   if (L >= g.nblocks) return;
-  const int v = L / g.nslab;
-  const int slab = L - v * g.nslab;
+  // workgroup L: vector v, z-chunk zci, slab of TY rows (slab fastest)
+  const int nvz = ZC ? g.nslab * g.nzc : g.nslab;
+  const int v = L / nvz;
+  const int zci = ZC ? (L - v * nvz) / g.nslab : 0;
+  const int slab = L - v * nvz - zci * g.nslab;
   const int j0 = slab * TY;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -433,6 +439,13 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
       zb = min(zb, g.zrange[2 * j]);
       ze = max(ze, g.zrange[2 * j + 1]);
     }
+  }
+  // this workgroup's chunk of the slab's z-range; planes past its end are
+  // computed by the march's last block but not stored (the next chunk's
+  // workgroup stores them)
+  if constexpr (ZC) {
+    zb = max(zb, zci * g.zc);
+    ze = min(ze, (zci + 1) * g.zc);
   }
   if (zb >= ze) return;
   const int zlo = zb - R;
@@ -498,7 +511,9 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
       const int po = off_plain(mo, c0 + c);
-      so[S][c] = off_masked(mo, c0 + c);
+      // planes past a chunk are not stored (past an unchunked slab they are empty)
+      if constexpr (ZC) so[S][c] = z < ze ? off_masked(mo, c0 + c) : OOB;
+      else so[S][c] = off_masked(mo, c0 + c);
       vq[S][c] = pc_bload<T>(rv, po);
       if constexpr (HASX) xq[S][c] = pc_bload<T>(rx, po);
     }
@@ -578,31 +593,61 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
 #define PC_LEAN_TY 8
 #endif
 
+// z-chunks for the lean and paired kernels: with few vectors the slab x vector
+// workgroups do not fill the chip (256 CUs x 32 wavefronts), so each slab's
+// z-range is split until the wavefronts would fill it once, keeping at least 8
+// planes per chunk (a chunk re-reads 2R queue planes of its neighbour and runs
+// a variant with a few more registers).  Measured (FP32, radius 4): H2O sphere
+// (69,264 points, k 23, 8 waves per workgroup) 30 -> 20 us; C10H8 sphere
+// (k 49, 16 waves per workgroup: 9,408 wavefronts already) 134 us unchunked,
+// 159-164 us when chunked to 1024-2048 workgroups.
+#ifndef PC_LEAN_ZCHUNK_WAVES
+#define PC_LEAN_ZCHUNK_WAVES 8192
+#endif
+inline void pc_zchunks(StArgs& a, int nz, int waves_per_wg) {
+  constexpr long long kWaves = PC_LEAN_ZCHUNK_WAVES;
+  const long long base = (long long)a.nslab * a.nvec * waves_per_wg;
+  int nzc = 1;
+  if (base < kWaves) nzc = (int)std::min<long long>((kWaves + base - 1) / base, std::max(1, nz / 8));
+  a.zc = (nz + nzc - 1) / nzc;
+  a.nzc = (nz + a.zc - 1) / a.zc;
+}
+
 template <typename T, int R, int SX>
 int launch_lean(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
                 double ca, double cb, double cc, hipStream_t st) {
   constexpr int TY = PC_LEAN_TY;
   constexpr int CPW = SX > 2 ? SX / 2 : 1;     // at most 16 waves (1024 threads) per workgroup
   constexpr int QL = pc_lean_ql(R);
+  constexpr int WPG = TY * SX / CPW;
   StArgs a;
   a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
   a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
   a.ld = ld;
   a.nslab = (gd->ny + TY - 1) / TY;
   a.nvec = nvec;
-  a.nblocks = a.nslab * nvec;
+  pc_zchunks(a, gd->nz, WPG);
+  a.nblocks = a.nslab * a.nzc * nvec;
   a.nb8 = (a.nblocks + 7) / 8;
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
   for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
   const dim3 grid(8 * a.nb8), block(64 * TY * SX / CPW);
   const size_t mbytes = sizeof(int2) * (size_t)(TY + 2 * R) * (size_t)(gd->nz + 2 * QL);
-  if (X)
-    hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, true>), grid, block, mbytes, st, a, Y, X, Out, V, T(ca),
-                       T(cb), T(cc));
-  else
-    hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, false>), grid, block, mbytes, st, a, Y, X, Out, V, T(ca),
-                       T(cb), T(cc));
+  if (a.nzc > 1) {
+    if (X)
+      hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, true, true>), grid, block, mbytes, st, a, Y, X, Out, V,
+                         T(ca), T(cb), T(cc));
+    else
+      hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, false, true>), grid, block, mbytes, st, a, Y, X, Out, V,
+                         T(ca), T(cb), T(cc));
+  } else if (X) {
+    hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, true, false>), grid, block, mbytes, st, a, Y, X, Out, V,
+                       T(ca), T(cb), T(cc));
+  } else {
+    hipLaunchKernelGGL((stencil_lean<T, R, SX, TY, CPW, false, false>), grid, block, mbytes, st, a, Y, X, Out, V,
+                       T(ca), T(cb), T(cc));
+  }
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
 }
@@ -645,8 +690,11 @@ void stencil_pair(StArgs g, const float* Y, const float* X, float* Out, const fl
   const int bid = blockIdx.x;
   const int L = (bid & 7) * g.nb8 + (bid >> 3);
   if (L >= g.nblocks) return;
-  const int v = L / g.nslab;
-  const int slab = L - v * g.nslab;
+  // workgroup L: vector v, z-chunk zci, slab of TY rows (slab fastest)
+  const int nvz = g.nslab * g.nzc;
+  const int v = L / nvz;
+  const int zci = (L - v * nvz) / g.nslab;
+  const int slab = L - v * nvz - zci * g.nslab;
   const int j0 = slab * TY;
   const int lane = threadIdx.x & 63;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -660,6 +708,8 @@ void stencil_pair(StArgs g, const float* Y, const float* X, float* Out, const fl
       ze = max(ze, g.zrange[2 * j + 1]);
     }
   }
+  zb = max(zb, zci * g.zc);
+  ze = min(ze, (zci + 1) * g.zc);
   if (zb >= ze) return;
   const int zlo = zb - R;
   const int zm = (ze - zb + QL - 1) / QL * QL + QL;
@@ -786,8 +836,9 @@ void stencil_pair(StArgs g, const float* Y, const float* X, float* Out, const fl
     if constexpr (HASX) o += cc * fix_pair(mo, xq[S]);
     {
       const int lo = mo.y & 0xffff, len = (mo.y >> 16) - lo;
-      const bool in0 = (unsigned)(i0 - lo) < (unsigned)len;
-      const bool in1 = (unsigned)(i0 + 1 - lo) < (unsigned)len;
+      const bool inz = z < ze;
+      const bool in0 = inz && (unsigned)(i0 - lo) < (unsigned)len;
+      const bool in1 = inz && (unsigned)(i0 + 1 - lo) < (unsigned)len;
       const int o2 = (in0 && in1) ? (mo.x + i0) * ES : OOB;
       const int o1 = (in0 != in1) ? (mo.x + i0 + (in0 ? 0 : 1)) * ES : OOB;
       __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pc_u2, o), ro, o2, 0, 0);
@@ -813,13 +864,15 @@ int launch_pair(const GridDev* gd, const float* Y, const float* X, float* Out, c
                 double scale, double ca, double cb, double cc, hipStream_t st) {
   constexpr int TY = PC_LEAN_TY;
   constexpr int QL = pc_lean_ql(R);
+  constexpr int WPG = TY;
   StArgs a;
   a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
   a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
   a.ld = ld;
   a.nslab = (gd->ny + TY - 1) / TY;
   a.nvec = nvec;
-  a.nblocks = a.nslab * nvec;
+  pc_zchunks(a, gd->nz, WPG);
+  a.nblocks = a.nslab * a.nzc * nvec;
   a.nb8 = (a.nblocks + 7) / 8;
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
@@ -851,6 +904,8 @@ int launch_cfg(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, in
   a.ld = ld;
   a.nslab = (gd->ny + TY - 1) / TY;
   a.nvec = nvec;
+  a.nzc = 1;
+  a.zc = gd->nz;
   a.nblocks = a.nslab * ((nvec + NV - 1) / NV);
   a.nb8 = (a.nblocks + 7) / 8;
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
@@ -1021,7 +1076,7 @@ int launch_direct_r(const GridDev* gd, const T* Y, const T* X, T* Out, const T* 
   StArgs a;
   a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
   a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
-  a.ld = ld; a.nslab = 0; a.nvec = nvec; a.nblocks = 0; a.nb8 = 0;
+  a.ld = ld; a.nslab = 0; a.nvec = nvec; a.nblocks = 0; a.nb8 = 0; a.nzc = 1; a.zc = gd->nz;
   for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
   a.w[0] = 3.0 * gd->w[0] * scale;
   for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
