@@ -103,8 +103,10 @@ int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
  * 128 <= k <= 320, one workgroup up to 512, larger k uses syevd), 3 = as 2
  * with the one-workgroup reduction always, 4 = as 2 with stedc always */
 int pc_dense_set_method(void* ctx, int method);
-/* how pc_dense_rr applies R^-1: 0 = trtri + trmm (default), 1 = trsm in
- * place (no counterpart in the reference: its QR gives W directly) */
+/* how pc_dense_rr forms the Cholesky factor and applies R^-1: 0 = rocSOLVER
+ * potrf + trtri + trmm, 1 = potrf + trsm in place, 2 = this library's blocked
+ * Cholesky factor and inverse (two launches) + GEMMs (default; k > 320 uses 0).
+ * No counterpart in the reference: its QR gives W directly. */
 int pc_dense_set_triangular(void* ctx, int mode);
 /* steps repeated with stedc because a cluster was too large (no counterpart
  * in the reference; for tests) */

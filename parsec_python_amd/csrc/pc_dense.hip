@@ -10,6 +10,7 @@
 #include "pc_common.h"
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
@@ -55,7 +56,10 @@ struct DenseCtx {
   TriWs tw;              // tridiagonal eigensolver workspace (kmax <= kTriMaxK)
   double* tri_lam;       // kmax: its eigenvalues before they replace the diagonal
   bool tri_ok;           // false after a cluster exceeded kTriMaxM (stedc from then on)
-  bool trsm;             // R^-1 applied by triangular solves (else trtri + trmm)
+  int tri_mode;          // R^-1: 0 = trtri + trmm, 1 = triangular solves, 2 = chol_blk_* + GEMMs (k <= 320)
+  double* cR;            // chol_blk_*: factor, inverse (32 nb x 32 nb each) and the diagonal blocks' inverses
+  double* cRi;
+  double* cD;
   int tri_fallbacks;     // steps repeated with stedc
 };
 
@@ -92,6 +96,226 @@ __global__ void zero_info(int* info) {
 
 __global__ void pack_out(const int* info, double* out, int k) {
   if (threadIdx.x < 4) out[k + threadIdx.x] = (double)info[threadIdx.x];
+}
+
+// ---- blocked Cholesky factor and its inverse (k <= 320), FP64 MFMA
+//
+// S = R^T R with R upper, in 32 x 32 blocks (S padded with the identity to
+// 32 nb, nb = ceil(k / 32)), then Ri = R^-1; both in column-major 32 nb x 32 nb
+// workspaces.  Replaces rocSOLVER potrf + trtri (about 30 launches at k = 275)
+// with two: chol_blk_factor (one workgroup, the blocks of R column by column:
+// R_ij = R_ii^-T (S_ij - sum_{l<i} R_li^T R_lj), the diagonal blocks factored
+// and inverted by one wavefront) and chol_blk_inverse (workgroup j: block
+// column j of R^-1 by block back substitution).  The block products run on
+// v_mfma_f64_16x16x4f64: each wavefront owns a 16 x 16 quarter of the 32 x 32
+// result (A[m][k] from lane m + 16 k, B[k][n] from lane n + 16 k, D[m][n] in
+// lane n + 16 (m % 4), register m / 4).
+constexpr int kCb = 32;            // block size
+constexpr int kCbMaxNb = 10;       // k <= 320
+constexpr int kCholMaxK = kCb * kCbMaxNb;
+constexpr int kCbLW = kCb + 1;     // LDS column stride of a block
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+struct CbLane {
+  int lm, lk, wm, wn;
+  __device__ CbLane() {
+    const int l = threadIdx.x & 63, w = threadIdx.x >> 6;
+    lm = l & 15; lk = l >> 4; wm = w >> 1; wn = w & 1;
+  }
+  // the element of the wavefront's quarter held in result register r
+  __device__ int row(int r) const { return 16 * wm + 4 * r + lk; }
+  __device__ int col() const { return 16 * wn + lm; }
+};
+
+__device__ __forceinline__ long long cb_blk(int i, int j, int ld) { return (long long)(kCb * j) * ld + kCb * i; }
+
+// acc += op(A) B over K = 32 (A, B column-major blocks; op(A) = A^T when TA)
+template <bool TA>
+__device__ __forceinline__ v4d cb_mma(v4d acc, const double* A, int lda, const double* B, int ldb, const CbLane& q) {
+  const int m = 16 * q.wm + q.lm, n = 16 * q.wn + q.lm;
+  double a[8], b[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    const int kk = 4 * s + q.lk;
+    a[s] = TA ? A[m * lda + kk] : A[kk * lda + m];
+    b[s] = B[n * ldb + kk];
+  }
+#pragma unroll
+  for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+  return acc;
+}
+
+// acc += sum_{u < nt} op(A[u]) B[u]; the operands of four terms are loaded
+// before their MFMAs, so four block reads are in flight at once
+template <bool TA>
+__device__ __forceinline__ v4d cb_mma_sum(v4d acc, const double* const* A, int lda, const double* const* B, int ldb,
+                                          int nt, const CbLane& q) {
+  const int m = 16 * q.wm + q.lm, n = 16 * q.wn + q.lm;
+  for (int u0 = 0; u0 < nt; u0 += 4) {
+    double a[4][8], b[4][8];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const bool in = u0 + u < nt;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        const int kk = 4 * s + q.lk;
+        a[u][s] = in ? (TA ? A[u0 + u][m * lda + kk] : A[u0 + u][kk * lda + m]) : 0.0;
+        b[u][s] = in ? B[u0 + u][n * ldb + kk] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int s = 0; s < 8; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[u][s], b[u][s], acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double cb_readlane(double v, int l) {
+  const long long b = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+
+// Cholesky factor and inverse of the symmetric 32 x 32 block in W (LDS), by
+// wavefront 0: lane t holds column t % 32 in registers and reads row c of R
+// from the lanes that hold it.  On success W holds R (upper, zero below) and
+// Di = R^-1 (upper, zero below).  Returns the 1-based column of the first
+// pivot that is not positive (or not finite), 0 on success.
+__device__ __forceinline__ int cb_chol32(double* W, double* Di) {
+  const int t = threadIdx.x & 31;
+  double w[kCb];
+#pragma unroll
+  for (int r = 0; r < kCb; ++r) w[r] = W[t * kCbLW + r];
+#pragma unroll
+  for (int c = 0; c < kCb; ++c) {
+    const double d = cb_readlane(w[c], c);
+    if (!(d > 0.0) || !(d < __builtin_inf())) return c + 1;
+    const double sq = __builtin_sqrt(d);
+    const double is = 1.0 / sq;
+    w[c] = t == c ? sq : w[c] * is;
+    // rows a > c of column t: W(a, t) -= R(c, a) R(c, t) (below the diagonal: unused)
+#pragma unroll
+    for (int a = c + 1; a < kCb; ++a) w[a] -= cb_readlane(w[c], a) * w[c];
+  }
+#pragma unroll
+  for (int r = 0; r < kCb; ++r) W[t * kCbLW + r] = r <= t ? w[r] : 0.0;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // column t of R^-1 by back substitution, R's columns read from LDS
+  double x[kCb];
+#pragma unroll
+  for (int r = 0; r < kCb; ++r) x[r] = r == t ? 1.0 : 0.0;
+#pragma unroll
+  for (int c = kCb - 1; c >= 0; --c) {
+    x[c] = x[c] / W[c * kCbLW + c];
+#pragma unroll
+    for (int i = 0; i < c; ++i) x[i] -= W[c * kCbLW + i] * x[c];
+  }
+#pragma unroll
+  for (int r = 0; r < kCb; ++r) Di[t * kCbLW + r] = x[r];
+  return 0;
+}
+
+// One workgroup of 256 threads.  R (ld 32 nb): the upper blocks of the factor
+// of S padded with the identity; Dinv (nb blocks, column-major 32 x 32): the
+// inverses of R's diagonal blocks; info[0]: as potrf's (0, or the 1-based
+// column of the first pivot that is not positive).
+__global__ __launch_bounds__(256) void chol_blk_factor(const double* __restrict__ S, int lds, int k, double* R,
+                                                       double* Dinv, int* info) {
+  __shared__ double W[kCb * kCbLW], Di[kCb * kCbLW];
+  __shared__ const double* pa[kCbMaxNb];
+  __shared__ const double* pb[kCbMaxNb];
+  __shared__ int fail;
+  const int nb = (k + kCb - 1) / kCb, KP = kCb * nb;
+  const CbLane q;
+  if (threadIdx.x == 0) {
+    fail = 0;
+    info[0] = 0;
+  }
+  for (int j = 0; j < nb; ++j) {
+    for (int i = 0; i <= j; ++i) {
+      double sv[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int gr = kCb * i + q.row(r), gc = kCb * j + q.col();
+        sv[r] = (gr < k && gc < k) ? S[(long long)gc * lds + gr] : (gr == gc ? 1.0 : 0.0);
+      }
+      if (threadIdx.x < i) {
+        pa[threadIdx.x] = R + cb_blk(threadIdx.x, i, KP);
+        pb[threadIdx.x] = R + cb_blk(threadIdx.x, j, KP);
+      }
+      __syncthreads();   // also: R blocks written in the previous steps are visible
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      acc = cb_mma_sum<true>(acc, pa, KP, pb, KP, i, q);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) W[q.col() * kCbLW + q.row(r)] = sv[r] - acc[r];
+      __syncthreads();
+      double* rij = R + cb_blk(i, j, KP);
+      if (i < j) {
+        v4d o = {0.0, 0.0, 0.0, 0.0};
+        o = cb_mma<true>(o, Dinv + (long long)i * kCb * kCb, kCb, W, kCbLW, q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) rij[(long long)q.col() * KP + q.row(r)] = o[r];
+      } else {
+        if (threadIdx.x < 64) {
+          const int f = cb_chol32(W, Di);
+          if (threadIdx.x == 0 && f != 0) {
+            fail = 1;
+            info[0] = kCb * j + f;
+          }
+        }
+        __syncthreads();
+        if (!fail) {
+          double* dj = Dinv + (long long)j * kCb * kCb;
+          for (int e = threadIdx.x; e < kCb * kCb; e += 256) {
+            const int a = e & (kCb - 1), b = e / kCb;
+            rij[(long long)b * KP + a] = W[b * kCbLW + a];
+            dj[e] = Di[b * kCbLW + a];
+          }
+        }
+      }
+      __syncthreads();
+      if (fail) return;
+    }
+  }
+}
+
+// Workgroup j (nb of them) writes block column j of Ri = R^-1 (ld 32 nb):
+// X_jj = Dinv_j, X_ij = -Dinv_i sum_{l=i+1..j} R_il X_lj for i = j-1 .. 0,
+// zero below the diagonal blocks.
+__global__ __launch_bounds__(256) void chol_blk_inverse(const double* __restrict__ R, const double* __restrict__ Dinv,
+                                                        int k, double* Ri) {
+  __shared__ double Y[kCb * kCbLW];
+  __shared__ const double* pa[kCbMaxNb];
+  __shared__ const double* pb[kCbMaxNb];
+  const int nb = (k + kCb - 1) / kCb, KP = kCb * nb;
+  const int j = blockIdx.x;
+  const CbLane q;
+  for (int e = threadIdx.x; e < kCb * kCb; e += 256) {
+    const int a = e & (kCb - 1), b = e / kCb;
+    Ri[cb_blk(j, j, KP) + (long long)b * KP + a] = Dinv[(long long)j * kCb * kCb + e];
+    for (int i = j + 1; i < nb; ++i) Ri[cb_blk(i, j, KP) + (long long)b * KP + a] = 0.0;
+  }
+  for (int i = j - 1; i >= 0; --i) {
+    if (threadIdx.x < j - i) {
+      const int l = i + 1 + threadIdx.x;
+      pa[threadIdx.x] = R + cb_blk(i, l, KP);
+      pb[threadIdx.x] = Ri + cb_blk(l, j, KP);
+    }
+    __syncthreads();   // also: the blocks X_lj written before are visible
+    v4d acc = {0.0, 0.0, 0.0, 0.0};
+    acc = cb_mma_sum<false>(acc, pa, KP, pb, KP, j - i, q);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Y[q.col() * kCbLW + q.row(r)] = acc[r];
+    __syncthreads();
+    v4d o = {0.0, 0.0, 0.0, 0.0};
+    o = cb_mma<false>(o, Dinv + (long long)i * kCb * kCb, kCb, Y, kCbLW, q);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) Ri[cb_blk(i, j, KP) + (long long)q.col() * KP + q.row(r)] = -o[r];
+    __syncthreads();
+  }
 }
 
 // ---- one-workgroup Householder tridiagonalization (LAPACK dsytrd layout, lower)
@@ -907,7 +1131,13 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipHostMalloc(&c->host_out, (size_t)(kmax + 4) * sizeof(double), hipHostMallocDefault));
   PC_HIP(hipMalloc(&c->sync, kTrdSyncInts * sizeof(int)));
   PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
-  c->trsm = false;
+  c->tri_mode = 2;
+  {
+    const int kb = std::min(kmax, kCholMaxK), nbb = (kb + kCb - 1) / kCb, kp = kCb * nbb;
+    PC_HIP(hipMalloc(&c->cR, (size_t)kp * kp * sizeof(double)));
+    PC_HIP(hipMalloc(&c->cRi, (size_t)kp * kp * sizeof(double)));
+    PC_HIP(hipMalloc(&c->cD, (size_t)nbb * kCb * kCb * sizeof(double)));
+  }
   c->tri_ok = kmax <= kTriMaxK;
   c->tri_fallbacks = 0;
   c->tri_lam = nullptr;
@@ -944,6 +1174,7 @@ int pc_dense_destroy(void* ctx) {
   (void)hipFree(c->tmp); (void)hipFree(c->E); (void)hipFree(c->out); (void)hipFree(c->info);
   (void)hipFree(c->resid); (void)hipFree(c->sweeps); (void)hipFree(c->tau); (void)hipFree(c->P);
   (void)hipFree(c->S0); (void)hipFree(c->G0);
+  (void)hipFree(c->cR); (void)hipFree(c->cRi); (void)hipFree(c->cD);
   if (c->tw.u1) { (void)hipFree(c->tw.u1); (void)hipFree(c->tw.piv); (void)hipFree(c->tw.lam); }
   (void)hipHostFree(c->host_out);
   delete c;
@@ -967,6 +1198,7 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   PC_CHECK_ARG(k >= 1 && k <= c->kmax, "k out of range");
   PC_CHECK_ARG(lds >= k && ldg >= k && ldt >= k, "leading dimension too small");
   const double one = 1.0;
+  const bool blk = c->tri_mode == 2 && k <= kCholMaxK;
   PC_HIP(hipMemcpy2DAsync(c->S0, k * sizeof(double), S, lds * sizeof(double), k * sizeof(double), k,
                           hipMemcpyDeviceToDevice, c->stream));
   PC_HIP(hipMemcpy2DAsync(c->G0, k * sizeof(double), Gy, ldg * sizeof(double), k * sizeof(double), k,
@@ -984,8 +1216,21 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   }
   hipLaunchKernelGGL(zero_info, dim3(1), dim3(64), 0, c->stream, c->info);
   PC_LAUNCH_CHECK();
+  if (blk) {
+    // R^-1 from chol_blk_*, then Gy <- R^-T (Gy R^-1) by two GEMMs
+    const int nbk = (k + kCb - 1) / kCb;
+    hipLaunchKernelGGL(chol_blk_factor, dim3(1), dim3(256), 0, c->stream, S, lds, k, c->cR, c->cD, c->info);
+    PC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(chol_blk_inverse, dim3(nbk), dim3(256), 0, c->stream, c->cR, c->cD, k, c->cRi);
+    PC_LAUNCH_CHECK();
+    const double zero = 0.0;
+    PC_RB(rocblas_dgemm(c->h, rocblas_operation_none, rocblas_operation_none, k, k, k, &one, Gy, ldg, c->cRi,
+                        kCb * nbk, &zero, c->tmp, k));
+    PC_RB(rocblas_dgemm(c->h, rocblas_operation_transpose, rocblas_operation_none, k, k, k, &one, c->cRi, kCb * nbk,
+                        c->tmp, k, &zero, Gy, ldg));
+  } else {
   PC_RB(rocsolver_dpotrf(c->h, rocblas_fill_upper, k, S, lds, c->info));
-  if (c->trsm) {
+  if (c->tri_mode == 1) {
     // Gy <- R^-T Gy R^-1 by two triangular solves in place
     PC_RB(rocblas_dtrsm(c->h, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
                         rocblas_diagonal_non_unit, k, k, &one, S, lds, Gy, ldg));
@@ -999,6 +1244,7 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
     // Gy = R^-T tmp
     PC_RB(rocblas_dtrmm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_transpose,
                         rocblas_diagonal_non_unit, k, k, &one, S, lds, c->tmp, k, Gy, ldg));
+  }
   }
   hipLaunchKernelGGL(symmetrize, dim3((k * k + 255) / 256), dim3(256), 0, c->stream, Gy, k, ldg);
   PC_LAUNCH_CHECK();
@@ -1061,7 +1307,12 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
                              c->info + 2));
     }
     // T = R^-1 Q
-    if (c->trsm) {
+    if (blk) {
+      const double zero = 0.0;
+      const int kp = kCb * ((k + kCb - 1) / kCb);
+      PC_RB(rocblas_dgemm(c->h, rocblas_operation_none, rocblas_operation_none, k, k, k, &one, c->cRi, kp, Q, ldq,
+                          &zero, T, ldt));
+    } else if (c->tri_mode == 1) {
       PC_HIP(hipMemcpy2DAsync(T, ldt * sizeof(double), Q, ldq * sizeof(double), k * sizeof(double), k,
                               hipMemcpyDeviceToDevice, c->stream));
       PC_RB(rocblas_dtrsm(c->h, rocblas_side_left, rocblas_fill_upper, rocblas_operation_none,
@@ -1150,13 +1401,14 @@ int pc_dense_trd_control(void* ctx, int spins, int reset, int* out_aborts) {
   return pc::PC_OK;
 }
 
-// How R^-1 is applied in pc_dense_rr: 0 = explicit inverse (trtri, then
-// three trmm), 1 = triangular solves in place (three trsm).
+// How the Cholesky factor and R^-1 are formed in pc_dense_rr: 0 = rocSOLVER
+// potrf + trtri, then three trmm; 1 = potrf, then three trsm in place; 2 =
+// chol_blk_factor + chol_blk_inverse and three GEMMs (default; k > 320 uses 0).
 int pc_dense_set_triangular(void* ctx, int mode) {
   DenseCtx* c = static_cast<DenseCtx*>(ctx);
   PC_CHECK_ARG(c, "null context");
-  PC_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (trtri + trmm) or 1 (trsm)");
-  c->trsm = mode == 1;
+  PC_CHECK_ARG(mode >= 0 && mode <= 2, "mode must be 0 (trtri + trmm), 1 (trsm) or 2 (blocked Cholesky + GEMM)");
+  c->tri_mode = mode;
   return pc::PC_OK;
 }
 

@@ -442,7 +442,7 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
   }
   // this workgroup's chunk of the slab's z-range; planes past its end are
   // computed by the march's last block but not stored (the next chunk's
-  // workgroup stores them)
+  // workgroup stores them; see the store range below)
   if constexpr (ZC) {
     zb = max(zb, zci * g.zc);
     ze = min(ze, (zci + 1) * g.zc);
@@ -457,7 +457,12 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
   const long long base = (long long)v * g.ld;
   const __amdgpu_buffer_rsrc_t ry = pc_rsrc(Y + base, nbytes);
   const __amdgpu_buffer_rsrc_t rx = pc_rsrc(HASX ? X + base : Y + base, nbytes);
-  const __amdgpu_buffer_rsrc_t ro = pc_rsrc(Out + base, nbytes);
+  // Points are stored plane by plane, so every point of planes >= ze lies at
+  // or past the first point of plane ze: a chunk's store range ends there and
+  // the planes its march computes past its end are dropped by the range check.
+  unsigned sbytes = nbytes;
+  if constexpr (ZC) sbytes = (unsigned)((ze < nz ? (long long)g.off[ny * ze] : g.npts) * ES);
+  const __amdgpu_buffer_rsrc_t ro = pc_rsrc(Out + base, sbytes);
   const __amdgpu_buffer_rsrc_t rv = pc_rsrc(V, nbytes);
 
   for (int t = threadIdx.x; t < 2 * LR * LW; t += 64 * NWV) (&tile[0][0])[t] = T(0);
@@ -511,9 +516,7 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
 #pragma unroll
     for (int c = 0; c < CPW; ++c) {
       const int po = off_plain(mo, c0 + c);
-      // planes past a chunk are not stored (past an unchunked slab they are empty)
-      if constexpr (ZC) so[S][c] = z < ze ? off_masked(mo, c0 + c) : OOB;
-      else so[S][c] = off_masked(mo, c0 + c);
+      so[S][c] = off_masked(mo, c0 + c);
       vq[S][c] = pc_bload<T>(rv, po);
       if constexpr (HASX) xq[S][c] = pc_bload<T>(rx, po);
     }
@@ -604,12 +607,16 @@ void stencil_lean(StArgs g, const T* Y, const T* X, T* Out, const T* V, T ca, T 
 #ifndef PC_LEAN_ZCHUNK_WAVES
 #define PC_LEAN_ZCHUNK_WAVES 8192
 #endif
+#ifndef PC_LEAN_FORCE_ZC
+#define PC_LEAN_FORCE_ZC 0
+#endif
 inline void pc_zchunks(StArgs& a, int nz, int waves_per_wg) {
   constexpr long long kWaves = PC_LEAN_ZCHUNK_WAVES;
   const long long base = (long long)a.nslab * a.nvec * waves_per_wg;
   int nzc = 1;
   if (base < kWaves) nzc = (int)std::min<long long>((kWaves + base - 1) / base, std::max(1, nz / 8));
   a.zc = (nz + nzc - 1) / nzc;
+  if (PC_LEAN_FORCE_ZC > 0) a.zc = std::min(nz, PC_LEAN_FORCE_ZC);
   a.nzc = (nz + a.zc - 1) / a.zc;
 }
 

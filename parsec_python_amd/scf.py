@@ -73,10 +73,15 @@ class SCFConfig:
     # 0.446 s.
     polish_deg: int | None = 4
     # eigensolver of the k x k Rayleigh-Ritz problem: "tridiag" (one-launch
-    # Householder tridiagonalization + divide and conquer, k <= 512; larger k
-    # falls back to "syevd"), "syevd" (rocSOLVER divide and conquer) or
-    # "syevj" (Jacobi).  All three return the same eigenpairs to rounding.
+    # Householder tridiagonalization + this library's tridiagonal eigensolver,
+    # k <= 512; larger k falls back to "syevd"), "syevd" (rocSOLVER divide and
+    # conquer) or "syevj" (Jacobi).  All three return the same eigenpairs to
+    # rounding.
     dense_method: str = "tridiag"
+    # Cholesky factor of the overlap matrix and its inverse: "blocked" (this
+    # library's two-launch blocked factor and inverse, then GEMMs; k <= 320,
+    # larger k uses "rocsolver") or "rocsolver" (potrf + trtri + trmm)
+    dense_cholesky: str = "blocked"
     # extra fixed-potential filter passes after first_filt when the SCF starts
     # from a given (learned) density, until the mean occupied Ritz value moves
     # by less than start_warmup_tol Ha (see run_scf).  Off by default: on
@@ -221,7 +226,9 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     H.dtype = prob.dtype
     eig = Eigensolver(H, k, seed=cfg.seed)
     dense_method = {"syevd": 0, "syevj": 1, "tridiag": 2}[cfg.dense_method]
+    tri_mode = {"rocsolver": 0, "blocked": 2}[cfg.dense_cholesky]
     _lib.call("pc_dense_set_method", eig.la.dense, dense_method)
+    _lib.call("pc_dense_set_triangular", eig.la.dense, tri_mode)
     hart = HartreeSolver(H.dgrid, h, reference_aliasing=cfg.pcg_aliasing, prec=cfg.pcg_prec)
     mixer = make_mixer(cfg.mixer, n, cfg.maxits + 2)
     if cfg.diagmeth not in (0, 1, 2, 3):
@@ -324,6 +331,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
             H.dtype = F64
             eig64 = Eigensolver(H, k, seed=cfg.seed)
             _lib.call("pc_dense_set_method", eig64.la.dense, dense_method)
+            _lib.call("pc_dense_set_triangular", eig64.la.dense, tri_mode)
             eig64.seeds = eig.seeds
             _lib.call("pc_convert_f32_to_f64", n, k, ptr(W), W.shape[1], ptr(eig64.bufs[0]), eig64.bufs[0].shape[1],
                       stream())

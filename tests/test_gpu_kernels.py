@@ -244,8 +244,10 @@ def test_dense_rr_methods(gpu, method, k, kind):
     _lib.call("pc_dense_set_method", la.dense, method)
     ref = eigh(G, S, eigvals_only=True)
     scale = max(1.0, np.abs(ref).max())
-    # R^-1 as trtri + trmm, and (method 2) also as triangular solves
-    for tri_mode in ((0, 1) if method == 2 else (0,)):
+    # R^-1 from the blocked Cholesky kernels (the default; rocSOLVER's potrf
+    # + trtri above k = 320), and (method 2) also as trtri + trmm and as
+    # triangular solves
+    for tri_mode in ((2, 0, 1) if method == 2 else (2,)):
         _lib.call("pc_dense_set_triangular", la.dense, tri_mode)
         Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
         Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
@@ -327,9 +329,10 @@ def test_dense_rr_large_cluster_falls_back_to_stedc(gpu):
 def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
     """Time of one Rayleigh-Ritz dense stage at k = 275 with this library's
     tridiagonal eigensolver (method 2) and with rocSOLVER's stedc (method 4),
-    and with R^-1 applied by triangular solves; all give the same pairs.
-    Printed for the record; the only bar is that the library's eigensolver is
-    not slower than stedc."""
+    with R^-1 from rocSOLVER's potrf + trtri, from triangular solves and from
+    the blocked Cholesky kernels; all give the same pairs.  Printed for the
+    record; the bars are that the library's eigensolver is not slower than
+    stedc and the blocked Cholesky kernels not slower than potrf + trtri."""
     import time
     import torch
     from parsec_python_amd import _lib
@@ -343,7 +346,7 @@ def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
     Sd0 = torch.tensor(S.T.copy(), device="cuda").view(-1)
     Gd0 = torch.tensor(G.T.copy(), device="cuda").view(-1)
     out = {}
-    for method, tri_mode in ((2, 0), (4, 0), (2, 1)):
+    for method, tri_mode in ((2, 0), (4, 0), (2, 1), (2, 2)):
         la = BlockLinalg(10, k)
         _lib.call("pc_dense_set_method", la.dense, method)
         _lib.call("pc_dense_set_triangular", la.dense, tri_mode)
@@ -356,11 +359,34 @@ def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
             ts.append(time.perf_counter() - t0)
         out[(method, tri_mode)] = (float(np.median(ts[1:])) * 1e3, lam)
         la.close()
-    print(f"  dense stage k {k}: tridiagonal eigensolver {out[(2, 0)][0]:.2f} ms, stedc {out[(4, 0)][0]:.2f} ms; "
-          f"with triangular solves {out[(2, 1)][0]:.2f} ms", flush=True)
-    for key in ((2, 0), (2, 1)):
+    print(f"  dense stage k {k}: tridiagonal eigensolver {out[(2, 0)][0]:.2f} ms, stedc {out[(4, 0)][0]:.2f} ms "
+          f"(potrf + trtri); with triangular solves {out[(2, 1)][0]:.2f} ms; with the blocked Cholesky kernels "
+          f"{out[(2, 2)][0]:.2f} ms", flush=True)
+    for key in ((2, 0), (2, 1), (2, 2)):
         assert np.allclose(out[key][1], out[(4, 0)][1], rtol=0, atol=1e-12 * k * np.abs(out[(4, 0)][1]).max())
     assert out[(2, 0)][0] <= 1.05 * out[(4, 0)][0]
+    assert out[(2, 2)][0] <= 1.05 * out[(2, 0)][0]
+
+
+@pytest.mark.parametrize("tri_mode", [0, 2])
+@pytest.mark.parametrize("k", [150, 275])
+def test_dense_rr_reports_indefinite_overlap(gpu, tri_mode, k):
+    """An overlap matrix with a negative pivot at column 101 that no shift of
+    at most 1e-9 of its largest diagonal entry repairs: both Cholesky forms
+    (rocSOLVER's potrf and the blocked kernels) report the same column, and
+    pc_dense_rr fails with that message after its three shifted retries."""
+    import torch
+    from parsec_python_amd import _lib
+    from parsec_python_amd._lib import ParsecHipError
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    S = np.eye(k)
+    S[100, 100] = -1.0
+    G = np.diag(np.arange(k, dtype=float))
+    la = BlockLinalg(10, k)
+    _lib.call("pc_dense_set_triangular", la.dense, tri_mode)
+    with pytest.raises(ParsecHipError, match="potrf info 101"):
+        la.dense_rr(torch.tensor(S.T.copy(), device="cuda").view(-1), torch.tensor(G.T.copy(), device="cuda").view(-1), k)
+    la.close()
 
 
 def test_density_xc_post_vs_oracle(gpu):
