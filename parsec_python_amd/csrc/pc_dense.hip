@@ -979,11 +979,28 @@ __device__ __forceinline__ double tri_block_sum(double v, double* red) {
   return r;
 }
 
+// Sum over the 256 threads of a workgroup: wavefront reductions, then the
+// four partials in a fixed order (the same value in every thread).
+__device__ __forceinline__ double tri_wg_sum(double v, double* red4) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = (red4[0] + red4[1]) + (red4[2] + red4[3]);
+  __syncthreads();
+  return r;
+}
+
 // Z (k x k, ld k) and the eigenvalues (ascending) into lam_out; info_flag
-// receives kTriClusterFlag when a cluster is larger than kTriMaxM.
+// receives kTriClusterFlag when a cluster is larger than kTriMaxM.  The first
+// workgroup of a cluster of m > 1 close eigenvalues copies the m inverse-
+// iteration vectors into LDS (dynamic, kTriMaxM * k doubles), orthonormalises
+// them by modified Gram-Schmidt twice (each thread owns the same rows
+// throughout, so only the reductions synchronise), forms C = Xc^T T Xc,
+// diagonalises it by cyclic Jacobi and writes Xc U in ascending order.
 __global__ __launch_bounds__(256) void tri_cluster(const double* __restrict__ d, const double* __restrict__ e, int k,
                                                    TriWs ws, double* Z, double* lam_out, int* info_flag) {
-  __shared__ double red[256];
+  extern __shared__ double xs[];
+  __shared__ double red4[4];
   __shared__ double C[kTriMaxM * kTriMaxM], U[kTriMaxM * kTriMaxM], wv[kTriMaxM];
   __shared__ int order[kTriMaxM];
   const int i = blockIdx.x;
@@ -992,9 +1009,8 @@ __global__ __launch_bounds__(256) void tri_cluster(const double* __restrict__ d,
   int j = i;
   while (j + 1 < k && ws.lam[j + 1] - ws.lam[j] < ortol) ++j;
   const int m = j - i + 1;
-  auto X = [&](int r, int c) -> double& { return ws.x[(long long)r * k + i + c]; };
   if (m == 1) {
-    for (int r = threadIdx.x; r < k; r += 256) Z[(long long)i * k + r] = X(r, 0);
+    for (int r = threadIdx.x; r < k; r += 256) Z[(long long)i * k + r] = ws.x[(long long)r * k + i];
     if (threadIdx.x == 0) lam_out[i] = ws.lam[i];
     return;
   }
@@ -1002,25 +1018,31 @@ __global__ __launch_bounds__(256) void tri_cluster(const double* __restrict__ d,
     if (threadIdx.x == 0) *info_flag = kTriClusterFlag;
     return;
   }
+  // the cluster's vectors into LDS, column c at xs + c k (global reads in runs of m)
+  for (int t = threadIdx.x; t < m * k; t += 256) {
+    const int r = t / m, c = t - r * m;
+    xs[c * k + r] = ws.x[(long long)r * k + i + c];
+  }
+  __syncthreads();
+  auto X = [&](int r, int c) -> double& { return xs[c * k + r]; };
   // modified Gram-Schmidt, twice
   for (int pass = 0; pass < 2; ++pass) {
     for (int a = 0; a < m; ++a) {
       for (int b = 0; b < a; ++b) {
         double s = 0.0;
         for (int r = threadIdx.x; r < k; r += 256) s += X(r, b) * X(r, a);
-        s = tri_block_sum(s, red);
+        s = tri_wg_sum(s, red4);
         for (int r = threadIdx.x; r < k; r += 256) X(r, a) -= s * X(r, b);
-        __syncthreads();
       }
       double s = 0.0;
       for (int r = threadIdx.x; r < k; r += 256) s += X(r, a) * X(r, a);
-      s = tri_block_sum(s, red);
+      s = tri_wg_sum(s, red4);
       const double inv = s > 0.0 ? 1.0 / sqrt(s) : 0.0;
       for (int r = threadIdx.x; r < k; r += 256) X(r, a) *= inv;
-      __syncthreads();
     }
   }
-  // C = Zc^T T Zc
+  __syncthreads();   // T Xc reads the neighbours' rows
+  // C = Xc^T T Xc
   for (int a = 0; a < m; ++a)
     for (int b = 0; b <= a; ++b) {
       double s = 0.0;
@@ -1030,7 +1052,7 @@ __global__ __launch_bounds__(256) void tri_cluster(const double* __restrict__ d,
         if (r + 1 < k) tz += e[r] * X(r + 1, b);
         s += X(r, a) * tz;
       }
-      s = tri_block_sum(s, red);
+      s = tri_wg_sum(s, red4);
       if (threadIdx.x == 0) {
         C[a * kTriMaxM + b] = s;
         C[b * kTriMaxM + a] = s;
@@ -1139,6 +1161,13 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
     PC_HIP(hipMalloc(&c->cD, (size_t)nbb * kCb * kCb * sizeof(double)));
   }
   c->tri_ok = kmax <= kTriMaxK;
+  if (c->tri_ok) {
+    // tri_cluster keeps a cluster's vectors in dynamic LDS: kTriMaxM * kmax doubles
+    const int lds_bytes = (int)((size_t)kTriMaxM * kmax * sizeof(double));
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&tri_cluster), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            lds_bytes) != hipSuccess)
+      c->tri_ok = false;
+  }
   c->tri_fallbacks = 0;
   c->tri_lam = nullptr;
   c->tw = TriWs{};
@@ -1288,8 +1317,8 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
         hipLaunchKernelGGL(tri_invit, dim3((k + kInvT - 1) / kInvT), dim3(kInvT), 0, c->stream, c->out, c->E, k,
                            c->tw);
         PC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(tri_cluster, dim3(k), dim3(256), 0, c->stream, c->out, c->E, k, c->tw, c->tmp,
-                           c->tri_lam, c->info + 2);
+        hipLaunchKernelGGL(tri_cluster, dim3(k), dim3(256), (size_t)kTriMaxM * k * sizeof(double), c->stream, c->out,
+                           c->E, k, c->tw, c->tmp, c->tri_lam, c->info + 2);
         PC_LAUNCH_CHECK();
         PC_HIP(hipMemcpyAsync(c->out, c->tri_lam, k * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
       } else {
