@@ -83,6 +83,15 @@ int pc_rotate_f64(const double* X, long long ldx, const double* Q, int ldq, doub
                   int kx, int ky, void* work, size_t work_bytes, void* stream);
 int pc_rotate_f32(const float* X, long long ldx, const double* Q, int ldq, float* Z, long long ldz, long long n,
                   int kx, int ky, void* work, size_t work_bytes, void* stream);
+/* workspace of pc_rotate_f32 (fp32 = 1) or pc_rotate_f64, including the split
+ * Q of the optional split-bf16 FP32 rotation.  No counterpart in the
+ * reference (its W @ Q allocates). */
+int pc_rotate_workspace(int kx, int ky, int fp32, size_t* out_bytes);
+/* process-wide switch of the FP32 rotation on bf16 MFMA with three-way operand
+ * splits (1 = on; off by default: faster, but it cost the Si87H76 SCF four
+ * iterations where the rotation matrix cancels).  No counterpart in the
+ * reference. */
+int pc_rotate_set_split_bf16(int on);
 
 /* ---- FP64 dense k x k stage on the GPU (rocSOLVER/rocBLAS) ------------------
  * replaces: scipy.linalg.qr + eigh in src/Eigensolvers/chefsi1.py:43,64 and

@@ -52,6 +52,8 @@ PROTOTYPES = {
     "pc_gram_f32": [_vp, _ll, _vp, _ll, _ll, _i, _i, _vp, _ll, _i, _vp, _sz, _vp],
     "pc_rotate_f64": [_vp, _ll, _vp, _i, _vp, _ll, _ll, _i, _i, _vp, _sz, _vp],
     "pc_rotate_f32": [_vp, _ll, _vp, _i, _vp, _ll, _ll, _i, _i, _vp, _sz, _vp],
+    "pc_rotate_workspace": [_i, _i, _i, _psz],
+    "pc_rotate_set_split_bf16": [_i],
     "pc_dense_create": [_vp, _i, C.POINTER(_vp)],
     "pc_dense_destroy": [_vp],
     "pc_dense_rr": [_vp, _i, _vp, _i, _vp, _i, _vp, _i, _vp, _pd],

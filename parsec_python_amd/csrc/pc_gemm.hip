@@ -470,6 +470,175 @@ __global__ __launch_bounds__(256, 2) void rotate_wide_f32(const float* __restric
     }
 }
 
+// FP32 rotation on bf16 MFMA with three-way splits: every FP32 operand value
+// x is split exactly into bf16 parts x = h + m + l (truncation: each part keeps
+// the next 8 significant bits), and Z = X Q takes the six products whose parts
+// carry at least 2^-16 of the value (lo*hi, mid*mid, hi*lo, mid*hi, hi*mid,
+// hi*hi, small terms first) on v_mfma_f32_16x16x32_bf16 with FP32
+// accumulation: on random operands the same accuracy as the FP32 MFMA path
+// (the dropped products are below 2^-24 relative) at 6 / 16 of its MFMA cost
+// (Si87H76 block, 605,808 x 275 x 275: 784 us against 1086 us).  Q (FP64) is rounded
+// to FP32 and split once by split_q_bf3 into slabs of 32 k's:
+// Qp[slab][column][part][32] (bf16 bits, columns padded to 32 NCH, zero
+// beyond kx / ky).  The 16x16x32 operand layouts are only relied on through
+// their shared K order: A (Q^T) lane m + 16 g element e and B (X^T) lane
+// n + 16 g element e both carry k = 8 g + e; D[m][n] in lane n + 16 (m / 4),
+// register m % 4, as for the other 16 x 16 FP32-output MFMAs.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr int kBf3K = 32;   // k's per slab
+
+__device__ __forceinline__ void split3(float x, unsigned& h, unsigned& m, unsigned& l) {
+  h = __builtin_bit_cast(unsigned, x) & 0xffff0000u;
+  const float r1 = x - __builtin_bit_cast(float, h);
+  m = __builtin_bit_cast(unsigned, r1) & 0xffff0000u;
+  l = __builtin_bit_cast(unsigned, r1 - __builtin_bit_cast(float, m));   // at most 8 significant bits
+}
+
+__global__ void split_q_bf3(const double* __restrict__ Q, int ldq, int kx, int ky, int nc, int nks,
+                            unsigned short* __restrict__ Qp) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nks * nc * kBf3K) return;
+  const int kk = (int)(t % kBf3K), col = (int)((t / kBf3K) % nc), sl = (int)(t / ((long long)kBf3K * nc));
+  const int k = kBf3K * sl + kk;
+  const float x = (k < kx && col < ky) ? (float)Q[(long long)col * ldq + k] : 0.0f;
+  unsigned h, m, l;
+  split3(x, h, m, l);
+  unsigned short* o = Qp + ((long long)sl * nc + col) * 3 * kBf3K + kk;
+  o[0] = (unsigned short)(h >> 16);
+  o[kBf3K] = (unsigned short)(m >> 16);
+  o[2 * kBf3K] = (unsigned short)(l >> 16);
+}
+
+// Workgroup: 8 waves (pw = point quarter, cw = column half) over 256 points
+// and all 32 NCH columns; a wave accumulates 64 points (4 interleaved tiles,
+// tile j = points 4 n + j, one float4 of X per vector and lane) x 16 NCH
+// columns.  Per slab of 32 k's: the Q parts through LDS (double-buffered),
+// X of the next slab loaded into registers while the MFMAs run.  Requires
+// 16-byte aligned X/Z columns (ld % 4 == 0).
+template <int NCH>
+__global__ __launch_bounds__(512) void rotate_bf3(const float* __restrict__ X, long long ldx,
+                                                  const unsigned short* __restrict__ Qp, float* __restrict__ Z,
+                                                  long long ldz, long long n, int kx, int ky, int nblocks, int nb8) {
+  constexpr int NC = 32 * NCH;
+  constexpr int SLAB = NC * 3 * kBf3K;   // bf16 values per slab
+  __shared__ __attribute__((aligned(16))) unsigned short qs[2][SLAB];
+  const int L = (blockIdx.x & 7) * nb8 + (blockIdx.x >> 3);
+  if (L >= nblocks) return;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int pw = w >> 1, cw = w & 1;
+  const int lm = lane & 15, g = lane >> 4;
+  const long long pl = (long long)L * 256 + pw * 64 + 4 * lm;
+  const int c0 = cw * 16 * NCH;
+  const int nks = (kx + kBf3K - 1) / kBf3K;
+  const bool full = pl + 3 < n;
+  v4f acc[NCH][4];
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[c][j] = v4f{0, 0, 0, 0};
+  auto qload = [&](int sl, int bf) {
+    const u32x4* src = reinterpret_cast<const u32x4*>(Qp + (long long)sl * SLAB);
+    u32x4* dst = reinterpret_cast<u32x4*>(qs[bf]);
+    for (int t = threadIdx.x; t < SLAB / 8; t += 512) dst[t] = src[t];
+  };
+  float xr[8][4];
+  auto xload = [&](int sl) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int kk = kBf3K * sl + 8 * g + e;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) xr[e][j] = 0.0f;
+      if (kk < kx) {
+        const float* xp = X + (long long)kk * ldx + pl;
+        if (full) {
+          const float4 t = *reinterpret_cast<const float4*>(xp);
+          xr[e][0] = t.x; xr[e][1] = t.y; xr[e][2] = t.z; xr[e][3] = t.w;
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j) xr[e][j] = pl + j < n ? xp[j] : 0.0f;
+        }
+      }
+    }
+  };
+  qload(0, 0);
+  xload(0);
+  __syncthreads();
+  for (int sl = 0; sl < nks; ++sl) {
+    const int bf = sl & 1;
+    bf16x8 bh[4], bm[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      unsigned ph[4], pm[4], pq[4];
+#pragma unroll
+      for (int e2 = 0; e2 < 4; ++e2) {
+        unsigned h0, m0, l0, h1, m1, l1;
+        split3(xr[2 * e2][j], h0, m0, l0);
+        split3(xr[2 * e2 + 1][j], h1, m1, l1);
+        // the high halves of two floats -> one bf16 pair (element 2 e2 in the low half)
+        ph[e2] = __builtin_amdgcn_perm(h1, h0, 0x07060302u);
+        pm[e2] = __builtin_amdgcn_perm(m1, m0, 0x07060302u);
+        pq[e2] = __builtin_amdgcn_perm(l1, l0, 0x07060302u);
+      }
+      bh[j] = __builtin_bit_cast(bf16x8, u32x4{ph[0], ph[1], ph[2], ph[3]});
+      bm[j] = __builtin_bit_cast(bf16x8, u32x4{pm[0], pm[1], pm[2], pm[3]});
+      bl[j] = __builtin_bit_cast(bf16x8, u32x4{pq[0], pq[1], pq[2], pq[3]});
+    }
+    const bool more = sl + 1 < nks;
+    if (more) {
+      xload(sl + 1);
+      qload(sl + 1, bf ^ 1);
+    }
+    const unsigned short* qb = qs[bf];
+#pragma unroll
+    for (int c = 0; c < NCH; ++c) {
+      const unsigned short* qa = qb + (c0 + 16 * c + lm) * 3 * kBf3K + 8 * g;
+      const bf16x8 ah = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(qa));
+      const bf16x8 am = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(qa + kBf3K));
+      const bf16x8 al = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(qa + 2 * kBf3K));
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[c][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm[j], acc[c][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[c][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh[j], acc[c][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm[j], acc[c][j], 0, 0, 0);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[c][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[c][j], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int c = 0; c < NCH; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int col = c0 + 16 * c + Acc<float>::row(g, r);
+      if (col >= ky) continue;
+      float* zr = Z + (long long)col * ldz + pl;
+      if (full) {
+        *reinterpret_cast<float4*>(zr) = make_float4(acc[c][0][r], acc[c][1][r], acc[c][2][r], acc[c][3][r]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (pl + j < n) zr[j] = acc[c][j][r];
+      }
+    }
+}
+
+size_t bf3_work_bytes(int kx, int ky) {
+  const size_t nks = (size_t)((kx + kBf3K - 1) / kBf3K), nc = 32 * (size_t)((ky + 31) / 32);
+  return nks * nc * 3 * kBf3K * sizeof(unsigned short);
+}
+// Off by default: on random operands it matches the FP32 MFMA rotation, but
+// in the Si87H76 SCF, where the rotation matrix R^-1 Q has large entries that
+// cancel, the mixed-precision run took 19 SCF iterations instead of 15 with it
+// (same energy to 3.6e-7 Ha/atom), so the 32-product sums inside the bf16 MFMA
+// lose more than the FP32 path's sequential accumulation there.
+bool g_rotate_bf3 = false;   // pc_rotate_set_split_bf16
+
 __global__ void convert_q_f32(const double* __restrict__ Q, int ldq, int kx, int ky, float* __restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= kx * ky) return;
@@ -556,16 +725,40 @@ int rotate_impl(const T* X, long long ldx, const double* Q, int ldq, T* Z, long 
   hipStream_t st = static_cast<hipStream_t>(stream);
   T* qt = static_cast<T*>(work);
   const int ne = kx * ky;
-  if constexpr (sizeof(T) == 4)
-    hipLaunchKernelGGL(convert_q_f32, dim3((ne + 255) / 256), dim3(256), 0, st, Q, ldq, kx, ky, qt);
-  else
-    hipLaunchKernelGGL(convert_q_f64, dim3((ne + 255) / 256), dim3(256), 0, st, Q, ldq, kx, ky, qt);
-  PC_LAUNCH_CHECK();
+  auto convert_q = [&]() -> int {
+    if constexpr (sizeof(T) == 4)
+      hipLaunchKernelGGL(convert_q_f32, dim3((ne + 255) / 256), dim3(256), 0, st, Q, ldq, kx, ky, qt);
+    else
+      hipLaunchKernelGGL(convert_q_f64, dim3((ne + 255) / 256), dim3(256), 0, st, Q, ldq, kx, ky, qt);
+    PC_LAUNCH_CHECK();
+    return pc::PC_OK;
+  };
   if constexpr (sizeof(T) == 4) {
     const bool vec = (ldx % 4 == 0) && (ldz % 4 == 0) && (reinterpret_cast<uintptr_t>(X) % 16 == 0) &&
                      (reinterpret_cast<uintptr_t>(Z) % 16 == 0);
     const int nch = (ky + 31) / 32;
+    if (vec && nch <= 9 && g_rotate_bf3 && work_bytes >= bf3_work_bytes(kx, ky)) {
+      unsigned short* qp = static_cast<unsigned short*>(work);
+      const int nks = (kx + kBf3K - 1) / kBf3K;
+      const long long nq = (long long)nks * 32 * nch * kBf3K;
+      hipLaunchKernelGGL(split_q_bf3, dim3((unsigned)((nq + 255) / 256)), dim3(256), 0, st, Q, ldq, kx, ky, 32 * nch,
+                         nks, qp);
+      PC_LAUNCH_CHECK();
+      const int nb = (int)((n + 255) / 256);
+      const int nb8 = (nb + 7) / 8;
+      switch (nch) {
+#define PC_ROTB(N)                                                                                                \
+  case N:                                                                                                         \
+    hipLaunchKernelGGL(rotate_bf3<N>, dim3(8 * nb8), dim3(512), 0, st, X, ldx, qp, Z, ldz, n, kx, ky, nb, nb8);    \
+    break;
+        PC_ROTB(1) PC_ROTB(2) PC_ROTB(3) PC_ROTB(4) PC_ROTB(5) PC_ROTB(6) PC_ROTB(7) PC_ROTB(8) PC_ROTB(9)
+#undef PC_ROTB
+      }
+      PC_LAUNCH_CHECK();
+      return pc::PC_OK;
+    }
     if (vec && nch <= 9) {
+      if (convert_q() != pc::PC_OK) return pc::PC_ERR_HIP;
       const int nb = (int)((n + 127) / 128);
       const int nb8 = (nb + 7) / 8;
       switch (nch) {
@@ -580,6 +773,7 @@ int rotate_impl(const T* X, long long ldx, const double* Q, int ldq, T* Z, long 
       return pc::PC_OK;
     }
   }
+  if (convert_q() != pc::PC_OK) return pc::PC_ERR_HIP;
   const int npt = (int)((n + 127) / 128);
   const int nct = (ky + 63) / 64;
   const int nb = npt * nct;
@@ -608,6 +802,18 @@ int pc_gram_f64(const double* X, long long ldx, const double* Y, long long ldy, 
 int pc_gram_f32(const float* X, long long ldx, const float* Y, long long ldy, long long n, int kx, int ky,
                 double* S, long long lds, int sym, void* work, size_t work_bytes, void* stream) {
   return gram_impl<float>(X, ldx, Y, ldy, n, kx, ky, S, lds, sym, work, work_bytes, stream);
+}
+
+int pc_rotate_workspace(int kx, int ky, int fp32, size_t* out_bytes) {
+  PC_CHECK_ARG(out_bytes && kx >= 1 && ky >= 1, "bad argument");
+  const size_t base = (size_t)kx * ky * (fp32 ? sizeof(float) : sizeof(double));
+  *out_bytes = fp32 ? std::max(base, bf3_work_bytes(kx, ky)) : base;
+  return pc::PC_OK;
+}
+
+int pc_rotate_set_split_bf16(int on) {
+  g_rotate_bf3 = on != 0;
+  return pc::PC_OK;
 }
 
 int pc_rotate_f64(const double* X, long long ldx, const double* Q, int ldq, double* Z, long long ldz, long long n,

@@ -131,11 +131,12 @@ class BlockLinalg:
 
     def rotate(self, X: torch.Tensor, Q: torch.Tensor, kx: int, ky: int, out: torch.Tensor) -> None:
         """out[:ky] = (X[:kx]^T Q)^T, i.e. Z = X Q with Q FP64 kx x ky column-major (ld kx)."""
-        esz = 8 if X.dtype == F64 else 4
-        work = self.scratch.bytes("rot", kx * ky * esz)
+        nbytes = C.c_size_t()
+        _lib.call("pc_rotate_workspace", kx, ky, int(X.dtype != F64), C.byref(nbytes))
+        work = self.scratch.bytes("rot", nbytes.value)
         suffix = "f64" if X.dtype == F64 else "f32"
         _lib.call(f"pc_rotate_{suffix}", ptr(X), X.shape[1], ptr(Q), kx, ptr(out), out.shape[1], self.n, kx, ky,
-                  ptr(work), kx * ky * esz, stream())
+                  ptr(work), nbytes.value, stream())
 
     def dense_rr(self, S: torch.Tensor, G: torch.Tensor, k: int):
         """Return (T (k x k column-major FP64 device), ritz values (host numpy))."""

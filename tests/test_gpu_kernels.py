@@ -175,11 +175,18 @@ def test_gram_and_rotate(gpu, dtype, n, k):
     assert np.max(np.abs(G.cpu().numpy().T - Gref)) <= tol * np.abs(Sref).max() * (1 if dtype == "f64" else 10)
     Q = rng.standard_normal((k + 3, k))  # column-major k x (k+3) -> row j = column j
     Qd = torch.tensor(Q, device="cuda")
-    Z = torch.zeros((k + 3, ld), dtype=tdt, device="cuda")
-    la.rotate(Xd, Qd.view(-1), k, k + 3, Z)
     Zref = (Xr.T @ Q.T).T
     ztol = 1e-12 if dtype == "f64" else 2e-6
-    assert np.max(np.abs(Z[:, :n].double().cpu().numpy() - Zref)) <= ztol * np.abs(Zref).max()
+    # FP32: the FP32 MFMA rotation (default) and the optional split-bf16 one, both to the FP32 bar
+    from parsec_python_amd import _lib
+    for split in ((0, 1) if dtype == "f32" else (0,)):
+        _lib.call("pc_rotate_set_split_bf16", split)
+        Z = torch.zeros((k + 3, ld), dtype=tdt, device="cuda")
+        la.rotate(Xd, Qd.view(-1), k, k + 3, Z)
+        err = np.max(np.abs(Z[:, :n].double().cpu().numpy() - Zref))
+        print(f"  rotate split_bf16={split}: max error {err / np.abs(Zref).max():.2e} relative", flush=True)
+        assert err <= ztol * np.abs(Zref).max()
+    _lib.call("pc_rotate_set_split_bf16", 0)
     la.close()
 
 
