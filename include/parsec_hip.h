@@ -44,6 +44,12 @@ int pc_stencil_f64(void* grid, const double* Y, const double* X, double* Out, co
                    long long ld, double scale, double a, double b, double c, double* dot_part, void* stream);
 int pc_stencil_f32(void* grid, const float* Y, const float* X, float* Out, const float* V, int nvec,
                    long long ld, double scale, double a, double b, double c, double* dot_part, void* stream);
+/* stencil kernel for radius-4 grids of at most 128 points per row
+ * (process-wide): 0 = fixed 64-point x-chunks per wavefront, 1 = the lanes
+ * dealt the slab's actual columns, 2 = the latter for FP32 launches of fewer
+ * than 128 vectors that fill the chip without z-chunks (default); the same
+ * results.  No counterpart in the reference. */
+int pc_stencil_set_variant(int variant);
 int pc_stencil_direct_f64(void* grid, const double* Y, const double* X, double* Out, const double* V, int nvec,
                           long long ld, double scale, double a, double b, double c, double* dot_part, void* stream);
 int pc_stencil_direct_f32(void* grid, const float* Y, const float* X, float* Out, const float* V, int nvec,

@@ -37,6 +37,7 @@ PROTOTYPES = {
     "pc_grid_destroy": [_vp],
     "pc_stencil_f64": [_vp, _vp, _vp, _vp, _vp, _i, _ll, _d, _d, _d, _d, _vp, _vp],
     "pc_stencil_f32": [_vp, _vp, _vp, _vp, _vp, _i, _ll, _d, _d, _d, _d, _vp, _vp],
+    "pc_stencil_set_variant": [_i],
     "pc_stencil_direct_f64": [_vp, _vp, _vp, _vp, _vp, _i, _ll, _d, _d, _d, _d, _vp, _vp],
     "pc_stencil_direct_f32": [_vp, _vp, _vp, _vp, _vp, _i, _ll, _d, _d, _d, _d, _vp, _vp],
     "pc_stencil_num_partials": [_vp, _i, _i, _pi, _pi],

@@ -41,6 +41,8 @@ struct GridDev {
   int* prow;     // [npts]: row (j + ny*k) of each domain point
   int radius;
   double w[8];   // 1D -d2/dx2 weights, centre first (unscaled)
+  int* cc_tab;   // stencil_cc: per slab of 8 rows, the own and halo columns (tile row << 16 | x), -1 padded
+  int cc_wmax;   // its wavefronts per workgroup (0: not built)
 };
 
 struct StArgs {
@@ -659,6 +661,174 @@ int launch_lean(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
   return pc::PC_OK;
 }
 
+// ---- the compacted-column z-march (one vector per workgroup) ----------------
+// stencil_lean gives each wavefront a fixed 64-point x-chunk of one row, so on a
+// sphere about half of its lanes hold no point.  Here the lanes of a workgroup
+// are dealt the slab's actual columns: the host lists, per slab of TY rows,
+// the columns (tile row, x) that any plane of the slab's march touches, row
+// after row (own rows first, then the 2R halo rows' columns for the halo
+// loads), and a workgroup has as many wavefronts as the larger list needs.
+// Each lane keeps the z-queue of its own column, loads one halo column per
+// plane, and reads its row metadata per plane from the LDS table (per lane:
+// the rows of one wavefront differ).  The arithmetic and its order are those
+// of stencil_lean, so the results are the same bits.
+constexpr int kCcTY = 8;
+
+template <typename T, int R, int NXP, bool HASX>
+__global__ __launch_bounds__(1024) void stencil_cc(StArgs g, const int* __restrict__ ctab, int wmax, const T* Y,
+                                                   const T* X, T* Out, const T* V, T ca, T cb, T cc) {
+  constexpr int TY = kCcTY;
+  constexpr int LW = NXP + 2 * R;
+  constexpr int LR = TY + 2 * R;
+  constexpr int NS = PC_LEAN_NS;
+  constexpr int QL = pc_lean_ql(R);
+  constexpr int ES = (int)sizeof(T);
+  constexpr int OOB = 0x7ffffff0;
+  __shared__ T tile[2][LR * LW];
+  extern __shared__ int2 mtab[];
+
+  const int bid = blockIdx.x;
+  const int L = (bid & 7) * g.nb8 + (bid >> 3);
+  if (L >= g.nblocks) return;
+  const int v = L / g.nslab;
+  const int slab = L - v * g.nslab;
+  const int j0 = slab * TY;
+  const int ny = g.ny, nz = g.nz;
+  int zb = nz, ze = 0;
+  for (int t = 0; t < TY; ++t) {
+    const int j = j0 + t;
+    if (j < ny) {
+      zb = min(zb, g.zrange[2 * j]);
+      ze = max(ze, g.zrange[2 * j + 1]);
+    }
+  }
+  if (zb >= ze) return;
+  const int zlo = zb - R;
+  const int zm = (ze - zb + QL - 1) / QL * QL + QL;
+
+  const unsigned nbytes = (unsigned)(g.npts * ES);
+  const long long base = (long long)v * g.ld;
+  const __amdgpu_buffer_rsrc_t ry = pc_rsrc(Y + base, nbytes);
+  const __amdgpu_buffer_rsrc_t rx = pc_rsrc(HASX ? X + base : Y + base, nbytes);
+  const __amdgpu_buffer_rsrc_t ro = pc_rsrc(Out + base, nbytes);
+  const __amdgpu_buffer_rsrc_t rv = pc_rsrc(V, nbytes);
+
+  // this lane's own column (tile row tr, x) and halo column (tile row th, x)
+  const int* cs = ctab + (long long)slab * 2 * wmax * 64;
+  const int co = cs[threadIdx.x], ch = cs[wmax * 64 + threadIdx.x];
+  const bool own = co >= 0, hal = ch >= 0;
+  const int tr = own ? (co >> 16) : R, xo = own ? (co & 0xffff) : 0;
+  const int th = hal ? (ch >> 16) : 0, xh = hal ? (ch & 0xffff) : 0;
+
+  for (int t = threadIdx.x; t < 2 * LR * LW; t += blockDim.x) (&tile[0][0])[t] = T(0);
+  for (int e = threadIdx.x; e < LR * zm; e += blockDim.x) {
+    const int t = e / zm, z = zlo + (e - t * zm);
+    const int j = j0 - R + t;
+    mtab[e] = (j >= 0 && j < ny && z >= 0 && z < nz) ? g.meta[j + ny * z] : make_int2(0, 0);
+  }
+  __syncthreads();
+
+  auto mrow = [&](int t, int z) -> int2 { return mtab[t * zm + (z - zlo)]; };
+  auto off_masked = [&](int2 m, int x, bool ok) -> int {
+    const int lo = m.y & 0xffff, len = (m.y >> 16) - lo;
+    return (ok && (unsigned)(x - lo) < (unsigned)len) ? (m.x + x) * ES : OOB;
+  };
+  auto off_plain = [&](int2 m, int x, bool ok) -> int { return ok ? (m.x + x) * ES : OOB; };
+
+  T q[QL];
+  T xq[NS], vq[NS], hq[NS];
+  int so[NS];
+#pragma unroll
+  for (int d = 0; d < QL; ++d) q[d] = pc_bload<T>(ry, off_masked(mrow(tr, zb - R + d), xo, own));
+  auto issue_plane = [&](auto set, int z) {
+    constexpr int S = decltype(set)::value;
+    const int2 mo = mrow(tr, z);
+    const int po = off_plain(mo, xo, own);
+    so[S] = off_masked(mo, xo, own);
+    vq[S] = pc_bload<T>(rv, po);
+    if constexpr (HASX) xq[S] = pc_bload<T>(rx, po);
+    hq[S] = pc_bload<T>(ry, off_masked(mrow(th, z), xh, hal));
+  };
+  pc_unroll_seq(std::make_integer_sequence<int, NS - 1>{}, [&](auto k) {
+    constexpr int K = decltype(k)::value;
+    issue_plane(std::integral_constant<int, K>{}, zb + K);
+  });
+
+  const T w0 = T(g.w[0]);
+  T wd[R + 1];
+#pragma unroll
+  for (int d = 1; d <= R; ++d) wd[d] = T(g.w[d]);
+
+  auto plane = [&](auto phase, int z) {
+    constexpr int P = decltype(phase)::value;
+    constexpr int S = P % NS;
+    constexpr int B = P & 1;
+    issue_plane(std::integral_constant<int, (P + NS - 1) % NS>{}, z + NS - 1);
+    __builtin_amdgcn_sched_barrier(0);
+    T* tb = tile[B];
+    if (own) tb[tr * LW + R + xo] = q[(P + R) % QL];
+    if (hal) tb[th * LW + R + xh] = hq[S];
+    lds_barrier();
+    const T* trow = tb + tr * LW + R + xo;
+    const T yc = q[(P + R) % QL];
+    T acc = w0 * yc;
+#pragma unroll
+    for (int d = 1; d <= R; ++d) {
+      const T nb = trow[-d] + trow[d] + trow[-d * LW] + trow[d * LW] + q[(P + R - d) % QL] + q[(P + R + d) % QL];
+      acc += wd[d] * nb;
+    }
+    T o = acc + (ca * vq[S] + cb) * yc;
+    if constexpr (HASX) o += cc * xq[S];
+    pc_bstore<T>(ro, so[S], o);
+    q[P] = pc_bload<T>(ry, off_masked(mrow(tr, z - R + QL), xo, own));
+    __builtin_amdgcn_sched_barrier(0);
+  };
+  for (int z0 = zb; z0 < ze; z0 += QL) {
+    pc_unroll_seq(std::make_integer_sequence<int, QL>{}, [&](auto ph) {
+      constexpr int P = decltype(ph)::value;
+      plane(ph, z0 + P);
+    });
+  }
+}
+
+// pc_stencil_set_variant: 0 = stencil_lean, 1 = stencil_cc, 2 = stencil_cc for
+// FP32 launches of fewer than 128 vectors that stencil_lean would not split
+// into z-chunks (the default).  Measured (FP32 step, radius 4, same bits):
+// C10H8 sphere (nx 92, 49 vectors) 118 us against 133 us; Si87H76 sphere
+// (nx 106, 275 vectors) 824 against 787 us; H2O sphere (nx 52, 23 vectors,
+// z-chunked by stencil_lean) 30 against 18 us.
+int g_stencil_cc = 2;
+
+template <typename T, int R>
+int launch_cc(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, int nvec, long long ld, double scale,
+              double ca, double cb, double cc, hipStream_t st) {
+  constexpr int TY = kCcTY;
+  constexpr int QL = pc_lean_ql(R);
+  StArgs a;
+  a.nx = gd->nx; a.ny = gd->ny; a.nz = gd->nz; a.npts = gd->npts;
+  a.lo = gd->lo; a.hi = gd->hi; a.off = gd->off; a.zrange = gd->zrange; a.meta = gd->meta;
+  a.ld = ld;
+  a.nslab = (gd->ny + TY - 1) / TY;
+  a.nvec = nvec;
+  a.nzc = 1; a.zc = gd->nz;
+  a.nblocks = a.nslab * nvec;
+  a.nb8 = (a.nblocks + 7) / 8;
+  for (int d = 0; d < 8; ++d) a.w[d] = 0.0;
+  a.w[0] = 3.0 * gd->w[0] * scale;
+  for (int d = 1; d <= R; ++d) a.w[d] = gd->w[d] * scale;
+  const dim3 grid(8 * a.nb8), block(64 * gd->cc_wmax);
+  const size_t mbytes = sizeof(int2) * (size_t)(TY + 2 * R) * (size_t)(gd->nz + 2 * QL);
+  if (gd->nx <= 64) {
+    if (X) hipLaunchKernelGGL((stencil_cc<T, R, 64, true>), grid, block, mbytes, st, a, gd->cc_tab, gd->cc_wmax, Y, X, Out, V, T(ca), T(cb), T(cc));
+    else hipLaunchKernelGGL((stencil_cc<T, R, 64, false>), grid, block, mbytes, st, a, gd->cc_tab, gd->cc_wmax, Y, X, Out, V, T(ca), T(cb), T(cc));
+  } else {
+    if (X) hipLaunchKernelGGL((stencil_cc<T, R, 128, true>), grid, block, mbytes, st, a, gd->cc_tab, gd->cc_wmax, Y, X, Out, V, T(ca), T(cb), T(cc));
+    else hipLaunchKernelGGL((stencil_cc<T, R, 128, false>), grid, block, mbytes, st, a, gd->cc_tab, gd->cc_wmax, Y, X, Out, V, T(ca), T(cb), T(cc));
+  }
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
 // ---- the paired z-march (FP32, rows of 65..128 points) ---------------------
 // stencil_lean with two adjacent x points per lane: lane l holds x = 2l, 2l+1,
 // so one wavefront covers a whole row and every Y / X / V access and the store
@@ -980,6 +1150,15 @@ int launch_r_sx(const GridDev* gd, const T* Y, const T* X, T* Out, const T* V, i
                           reinterpret_cast<float*>(Out), reinterpret_cast<const float*>(V), nvec, ld, scale, ca, cb,
                           cc, st);
 #endif
+  if constexpr (SX <= 2 && R == 4) {
+    bool use_cc = g_stencil_cc == 1;
+    if (g_stencil_cc == 2 && sizeof(T) == 4 && nvec < 128) {
+      constexpr int CPW = SX > 2 ? SX / 2 : 1;
+      const long long lean_waves = (long long)((gd->ny + PC_LEAN_TY - 1) / PC_LEAN_TY) * nvec * (PC_LEAN_TY * SX / CPW);
+      use_cc = lean_waves >= PC_LEAN_ZCHUNK_WAVES;
+    }
+    if (use_cc && gd->cc_wmax > 0) return launch_cc<T, R>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, st);
+  }
 #ifndef PC_STENCIL_NO_LEAN
   if constexpr (SX <= 2) return launch_lean<T, R, SX>(gd, Y, X, Out, V, nvec, ld, scale, ca, cb, cc, st);
 #endif
@@ -1264,6 +1443,45 @@ int pc_grid_create(int nx, int ny, int nz, const int* row_lo, const int* row_hi,
   for (int r = 0; r < nrow; ++r) meta[r] = make_int2(off[r] - lo[r], lo[r] | (hi[r] << 16));
   GridDev* g = new GridDev();
   g->nx = nx; g->ny = ny; g->nz = nz; g->npts = cnt; g->radius = radius;
+  g->cc_tab = nullptr;
+  g->cc_wmax = 0;
+  // stencil_cc column lists: per slab of kCcTY rows, each own row's and halo
+  // row's x-range over the planes of the slab's march (an interval on a convex
+  // domain), as (tile row << 16 | x)
+  std::vector<int> cc;
+  int ccw = 0;
+  if (nx <= 128) {
+    const int TY = kCcTY, nslab = (ny + TY - 1) / TY, LR = TY + 2 * radius;
+    std::vector<std::vector<int>> own(nslab), hal(nslab);
+    for (int sl = 0; sl < nslab; ++sl) {
+      int zb = nz, ze = 0;
+      for (int t = 0; t < TY; ++t) {
+        const int j = sl * TY + t;
+        if (j < ny) { zb = std::min(zb, zr[2 * j]); ze = std::max(ze, zr[2 * j + 1]); }
+      }
+      for (int t = 0; t < LR; ++t) {
+        const int j = sl * TY - radius + t;
+        if (j < 0 || j >= ny || zb >= ze) continue;
+        int xl = nx, xh = 0;
+        for (int k = zb; k < ze; ++k) {
+          const int r = j + ny * k;
+          if (hi[r] > lo[r]) { xl = std::min(xl, lo[r]); xh = std::max(xh, hi[r]); }
+        }
+        const bool is_own = t >= radius && t < radius + TY;
+        for (int x = xl; x < xh; ++x) (is_own ? own[sl] : hal[sl]).push_back((t << 16) | x);
+      }
+      ccw = std::max(ccw, (int)std::max((own[sl].size() + 63) / 64, (hal[sl].size() + 63) / 64));
+    }
+    if (ccw >= 1 && ccw <= 16) {
+      cc.assign((size_t)nslab * 2 * ccw * 64, -1);
+      for (int sl = 0; sl < nslab; ++sl) {
+        std::copy(own[sl].begin(), own[sl].end(), cc.begin() + (size_t)sl * 2 * ccw * 64);
+        std::copy(hal[sl].begin(), hal[sl].end(), cc.begin() + (size_t)sl * 2 * ccw * 64 + ccw * 64);
+      }
+    } else {
+      ccw = 0;
+    }
+  }
   for (int d = 0; d < 8; ++d) g->w[d] = d <= radius ? weights[d] : 0.0;
   hipError_t e = hipSuccess;
   e = hipMalloc(&g->lo, nrow * sizeof(int));
@@ -1281,6 +1499,11 @@ int pc_grid_create(int nx, int ny, int nz, const int* row_lo, const int* row_hi,
   if (e == hipSuccess) e = hipMemcpy(g->hi, hi.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->off, off.data(), nrow * sizeof(int), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(g->zrange, zr.data(), 2 * ny * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && ccw > 0) {
+    e = hipMalloc(&g->cc_tab, cc.size() * sizeof(int));
+    if (e == hipSuccess) e = hipMemcpy(g->cc_tab, cc.data(), cc.size() * sizeof(int), hipMemcpyHostToDevice);
+    if (e == hipSuccess) g->cc_wmax = ccw;
+  }
   if (e != hipSuccess) {
     pc::set_error(std::string("pc_grid_create: ") + hipGetErrorString(e));
     return pc::PC_ERR_HIP;
@@ -1295,7 +1518,17 @@ int pc_grid_destroy(void* grid) {
   if (!g) return pc::PC_OK;
   (void)hipFree(g->lo); (void)hipFree(g->hi); (void)hipFree(g->off); (void)hipFree(g->zrange);
   (void)hipFree(g->meta); (void)hipFree(g->prow);
+  if (g->cc_tab) (void)hipFree(g->cc_tab);
   delete g;
+  return pc::PC_OK;
+}
+
+// Stencil kernel for radius-4 grids of at most 128 points per row
+// (process-wide): 0 = stencil_lean, 1 = stencil_cc, 2 = stencil_cc where it
+// measured faster (the default; see g_stencil_cc).
+int pc_stencil_set_variant(int variant) {
+  PC_CHECK_ARG(variant >= 0 && variant <= 2, "variant must be 0, 1 or 2");
+  g_stencil_cc = variant;
   return pc::PC_OK;
 }
 

@@ -73,6 +73,44 @@ def test_stencil_step_vs_oracle(gpu, nx, ny, nz, kind, radius, dtype, nvec):
     H.close()
 
 
+@pytest.mark.parametrize("nx,ny,nz,kind", [(20, 20, 20, "cube"), (21, 21, 21, "sphere"), (70, 12, 9, "cube"),
+                                             (67, 67, 67, "sphere"), (106, 106, 106, "sphere")])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+@pytest.mark.parametrize("nvec", [9, 17])
+def test_stencil_compacted_columns_match_lean(gpu, nx, ny, nz, kind, dtype, nvec):
+    """The compacted-column z-march (lanes dealt the slab's actual columns)
+    gives the same bits as the fixed x-chunk one, for the recurrence step and
+    the first step (no X), radius 4."""
+    import torch
+    from parsec_python_amd import _lib
+    from parsec_python_amd.hamiltonian import Hamiltonian, leading_dim
+    print(f"stencil cc case {nx}x{ny}x{nz} {kind} {dtype} nvec={nvec}", flush=True)
+    g = _grid(nx, ny, nz, 0.45, kind)
+    n = g.npts
+    rng = np.random.default_rng(nx + 5 * nvec)
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    H = Hamiltonian(g, 4, None, torch.tensor(rng.standard_normal(n), device="cuda"), tdt)
+    ld = leading_dim(n)
+    Y = torch.zeros((nvec, ld), dtype=tdt, device="cuda")
+    X = torch.zeros((nvec, ld), dtype=tdt, device="cuda")
+    Y[:, :n] = torch.tensor(rng.standard_normal((nvec, n)), dtype=tdt)
+    X[:, :n] = torch.tensor(rng.standard_normal((nvec, n)), dtype=tdt)
+    outs = {}
+    try:
+        for variant in (0, 1):
+            _lib.call("pc_stencil_set_variant", variant)
+            o1 = torch.zeros_like(X)
+            o2 = torch.zeros_like(X)
+            H.step(Y, X, o1, nvec, 0.7, -0.3, -1.1)
+            H.step(Y, None, o2, nvec, 0.7, -0.3, 0.0)
+            outs[variant] = (o1[:, :n].clone(), o2[:, :n].clone())
+    finally:
+        _lib.call("pc_stencil_set_variant", 2)
+    assert torch.equal(outs[0][0], outs[1][0])
+    assert torch.equal(outs[0][1], outs[1][1])
+    H.close()
+
+
 @pytest.mark.parametrize("entry", ["pc_stencil_f64", "pc_stencil_direct_f64"])
 def test_stencil_dot_partials(gpu, entry):
     """Per-workgroup partial sums of out*y (vector-major) add up to the oracle's dots."""
