@@ -132,7 +132,7 @@ int pc_dense_tri_fallbacks(void* ctx, int* out);
  * No counterpart in the reference. */
 int pc_dense_trd_control(void* ctx, int spins, int reset, int* out_aborts);
 /* pc_dense_rr repeats a step whose overlap matrix is not numerically positive
- * definite with S + shift*max(diag S)*I, shift 1e-13, 1e-11, 1e-9 (the
+ * definite with S + shift*max(diag S)*I, shift 1e-13, 1e-11, 1e-9, 1e-7 (the
  * reference's Householder QR in src/Eigensolvers/chefsi1.py:43 does not fail on
  * nearly dependent columns); out = steps of this context that needed it. */
 int pc_dense_shift_count(void* ctx, int* out);

@@ -57,6 +57,7 @@ struct DenseCtx {
   double* tri_lam;       // kmax: its eigenvalues before they replace the diagonal
   bool tri_ok;           // false after a cluster exceeded kTriMaxM (stedc from then on)
   int tri_mode;          // R^-1: 0 = trtri + trmm, 1 = triangular solves, 2 = chol_blk_* + GEMMs (k <= 320)
+  int blk_fallbacks;     // attempts repeated with potrf after chol_blk_factor found a pivot that is not positive
   double* cR;            // chol_blk_*: factor, inverse (32 nb x 32 nb each) and the diagonal blocks' inverses
   double* cRi;
   double* cD;
@@ -895,7 +896,11 @@ __global__ __launch_bounds__(kInvT) void tri_invit(const double* __restrict__ d,
       cu = -m * nu;
     }
   }
+  // the last row has no superdiagonals; the back substitution still reads them
+  // (times zero), so they must be zeros, not whatever the workspace held
   ws.u1[at(k - 1)] = cd;
+  ws.u2[at(k - 1)] = 0.0;
+  ws.u3[at(k - 1)] = 0.0;
   // fixed pseudo-random start vector in [-0.5, 0.5) (no two eigenvalues start alike)
   for (int j = 0; j < k; ++j) {
     unsigned h = (unsigned)j * 0x9E3779B1u ^ (unsigned)(i + 1) * 0x85EBCA77u;
@@ -1154,6 +1159,7 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipMalloc(&c->sync, kTrdSyncInts * sizeof(int)));
   PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
   c->tri_mode = 2;
+  c->blk_fallbacks = 0;
   {
     const int kb = std::min(kmax, kCholMaxK), nbb = (kb + kCb - 1) / kCb, kp = kCb * nbb;
     PC_HIP(hipMalloc(&c->cR, (size_t)kp * kp * sizeof(double)));
@@ -1217,7 +1223,7 @@ int pc_dense_destroy(void* ctx) {
 // When the overlap matrix is not numerically positive definite (a block whose
 // columns are nearly dependent, as the Ritz vectors of a Lanczos basis without
 // reorthogonalisation can be), the step is repeated from saved copies of S
-// and Gy with S + shift * max_i S_ii I, shift = 1e-13, 1e-11, 1e-9: the
+// and Gy with S + shift * max_i S_ii I, shift = 1e-13, 1e-11, 1e-9, 1e-7: the
 // dependent directions then get large Ritz values and leave the wanted lower
 // end of the spectrum, which the reference's Householder QR leaves in place.
 int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, double* T, int ldt, double* lam_dev,
@@ -1227,20 +1233,27 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   PC_CHECK_ARG(k >= 1 && k <= c->kmax, "k out of range");
   PC_CHECK_ARG(lds >= k && ldg >= k && ldt >= k, "leading dimension too small");
   const double one = 1.0;
-  const bool blk = c->tri_mode == 2 && k <= kCholMaxK;
+  // the blocked Cholesky kernels, until they report a pivot that is not
+  // positive: that attempt is repeated with rocSOLVER's potrf (the same
+  // shift), which then serves the rest of the call
+  bool blk = c->tri_mode == 2 && k <= kCholMaxK;
   PC_HIP(hipMemcpy2DAsync(c->S0, k * sizeof(double), S, lds * sizeof(double), k * sizeof(double), k,
                           hipMemcpyDeviceToDevice, c->stream));
   PC_HIP(hipMemcpy2DAsync(c->G0, k * sizeof(double), Gy, ldg * sizeof(double), k * sizeof(double), k,
                           hipMemcpyDeviceToDevice, c->stream));
-  constexpr int kShiftTries = 3;
+  constexpr int kShiftTries = 4;
+  constexpr double kShifts[kShiftTries + 1] = {0.0, 1e-13, 1e-11, 1e-9, 1e-7};
+  bool first = true;
   for (int shift_try = 0;; ++shift_try) {
-  if (shift_try > 0) {
+  if (!first) {
     PC_HIP(hipMemcpy2DAsync(S, lds * sizeof(double), c->S0, k * sizeof(double), k * sizeof(double), k,
                             hipMemcpyDeviceToDevice, c->stream));
     PC_HIP(hipMemcpy2DAsync(Gy, ldg * sizeof(double), c->G0, k * sizeof(double), k * sizeof(double), k,
                             hipMemcpyDeviceToDevice, c->stream));
-    const double shift = shift_try == 1 ? 1e-13 : (shift_try == 2 ? 1e-11 : 1e-9);
-    hipLaunchKernelGGL(add_diag_shift, dim3(1), dim3(256), 0, c->stream, S, k, lds, shift);
+  }
+  first = false;
+  if (shift_try > 0) {
+    hipLaunchKernelGGL(add_diag_shift, dim3(1), dim3(256), 0, c->stream, S, k, lds, kShifts[shift_try]);
     PC_LAUNCH_CHECK();
   }
   hipLaunchKernelGGL(zero_info, dim3(1), dim3(64), 0, c->stream, c->info);
@@ -1377,6 +1390,12 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
     break;
   }
   const int i0 = (int)c->host_out[k];
+  if (i0 != 0 && blk) {
+    blk = false;
+    c->blk_fallbacks += 1;
+    --shift_try;
+    continue;
+  }
   if (i0 != 0 && shift_try < kShiftTries) continue;
   if (shift_try > 0) c->shifted += 1;
   break;
@@ -1384,7 +1403,7 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   const int i0 = (int)c->host_out[k], i1 = (int)c->host_out[k + 1], i2 = (int)c->host_out[k + 2];
   if (i0 != 0) {
     pc::set_error("pc_dense_rr: overlap matrix not positive definite (potrf info " + std::to_string(i0) +
-                  ") even with a shift of 1e-9 of its largest diagonal entry");
+                  ") even with a shift of 1e-7 of its largest diagonal entry");
     return pc::PC_ERR_SOLVER;
   }
   if (i1 != 0 || i2 != 0) {

@@ -413,13 +413,42 @@ def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
     assert out[(2, 2)][0] <= 1.05 * out[(2, 0)][0]
 
 
+@pytest.mark.parametrize("k", [34, 275])
+def test_dense_rr_after_nan_filled_device_memory(gpu, k):
+    """Device memory released by earlier work and handed to a new dense
+    context holds whatever was there (here NaN): the Rayleigh-Ritz stage must
+    read none of it before writing (the inverse iteration once read the last
+    row's superdiagonals of its LU workspace unwritten)."""
+    import torch
+    from scipy.linalg import eigh
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    junk = torch.full((96 * 1024 * 1024,), float("nan"), dtype=torch.float64, device="cuda")
+    del junk
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    rng = np.random.default_rng(3 + k)
+    B = rng.standard_normal((2 * k, k))
+    S = B.T @ B / B.shape[0] + 0.5 * np.eye(k)
+    M = rng.standard_normal((k, k))
+    G = M + M.T
+    la = BlockLinalg(10, k)
+    T, lam = la.dense_rr(torch.tensor(S.T.copy(), device="cuda").view(-1),
+                         torch.tensor(G.T.copy(), device="cuda").view(-1), k)
+    Tm = T.cpu().numpy()[:k * k].reshape(k, k).T
+    assert np.all(np.isfinite(lam)) and np.all(np.isfinite(Tm))
+    assert np.allclose(lam, eigh(G, S, eigvals_only=True), rtol=0, atol=1e-11 * k * max(1.0, np.abs(lam).max()))
+    assert np.abs(Tm.T @ S @ Tm - np.eye(k)).max() < 1e-10 * k
+    la.close()
+
+
 @pytest.mark.parametrize("tri_mode", [0, 2])
 @pytest.mark.parametrize("k", [150, 275])
 def test_dense_rr_reports_indefinite_overlap(gpu, tri_mode, k):
     """An overlap matrix with a negative pivot at column 101 that no shift of
-    at most 1e-9 of its largest diagonal entry repairs: both Cholesky forms
-    (rocSOLVER's potrf and the blocked kernels) report the same column, and
-    pc_dense_rr fails with that message after its three shifted retries."""
+    at most 1e-7 of its largest diagonal entry repairs: both Cholesky forms
+    (rocSOLVER's potrf and the blocked kernels, whose failed attempt is
+    repeated with potrf) report the same column, and pc_dense_rr fails with
+    that message after its four shifted retries."""
     import torch
     from parsec_python_amd import _lib
     from parsec_python_amd._lib import ParsecHipError
