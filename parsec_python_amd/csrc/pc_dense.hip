@@ -219,67 +219,129 @@ __device__ __forceinline__ int cb_chol32(double* W, double* Di) {
   return 0;
 }
 
-// One workgroup of 256 threads.  R (ld 32 nb): the upper blocks of the factor
-// of S padded with the identity; Dinv (nb blocks, column-major 32 x 32): the
-// inverses of R's diagonal blocks; info[0]: as potrf's (0, or the 1-based
-// column of the first pivot that is not positive).
+// acc += op(A) B over K = 32 for a whole 32 x 32 block by one wavefront: four
+// 16 x 16 tiles, acc[tm][tn] (A, B column-major; op(A) = A^T when TA)
+template <bool TA>
+__device__ __forceinline__ void cb_mma_block(v4d (&acc)[2][2], const double* A, int lda, const double* B, int ldb,
+                                             double sgn) {
+  const int lane = threadIdx.x & 63, lm = lane & 15, lk = lane >> 4;
+  double a[2][8], b[2][8];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      const int kk = 4 * s + lk, m = 16 * t + lm;
+      a[t][s] = sgn * (TA ? A[m * lda + kk] : A[kk * lda + m]);
+      b[t][s] = B[m * ldb + kk];
+    }
+#pragma unroll
+  for (int s = 0; s < 8; ++s)
+#pragma unroll
+    for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+      for (int tn = 0; tn < 2; ++tn)
+        acc[tm][tn] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[tm][s], b[tn][s], acc[tm][tn], 0, 0, 0);
+}
+
+// R (ld 32 nb) <- the upper blocks of S padded with the identity, one thread
+// per element (a launch of its own: in the one-workgroup factor kernel the
+// copy's load latencies add up to most of its time)
+__global__ void chol_blk_init(const double* __restrict__ S, int lds, int k, double* __restrict__ R) {
+  const int nb = (k + kCb - 1) / kCb, KP = kCb * nb;
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= KP * KP) return;
+  const int r = e % KP, c = e / KP;
+  if (r / kCb > c / kCb) return;
+  R[(long long)c * KP + r] = (r < k && c < k) ? S[(long long)c * lds + r] : (r == c ? 1.0 : 0.0);
+}
+
+// One workgroup of 256 threads, right-looking by block columns: R (ld 32 nb)
+// holds the upper blocks of S padded with the identity (chol_blk_init); for each block
+// column j the diagonal block is factored and inverted by wavefront 0
+// (cb_chol32), then the wavefronts share out the panel blocks
+// R_jc = R_jj^-T A_jc (c > j) and afterwards the trailing blocks
+// A_ab -= R_ja^T R_jb (j < a <= b), one whole 32 x 32 block per wavefront at a
+// time: three dependent phases per block column.  Dinv (nb blocks,
+// column-major 32 x 32): the inverses of R's diagonal blocks; info[0]: as
+// potrf's (0, or the 1-based column of the first pivot that is not positive).
 __global__ __launch_bounds__(256) void chol_blk_factor(const double* __restrict__ S, int lds, int k, double* R,
                                                        double* Dinv, int* info) {
   __shared__ double W[kCb * kCbLW], Di[kCb * kCbLW];
-  __shared__ const double* pa[kCbMaxNb];
-  __shared__ const double* pb[kCbMaxNb];
   __shared__ int fail;
   const int nb = (k + kCb - 1) / kCb, KP = kCb * nb;
-  const CbLane q;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int lm = lane & 15, lk = lane >> 4;
   if (threadIdx.x == 0) {
     fail = 0;
     info[0] = 0;
   }
+  __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    for (int i = 0; i <= j; ++i) {
-      double sv[4];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int gr = kCb * i + q.row(r), gc = kCb * j + q.col();
-        sv[r] = (gr < k && gc < k) ? S[(long long)gc * lds + gr] : (gr == gc ? 1.0 : 0.0);
-      }
-      if (threadIdx.x < i) {
-        pa[threadIdx.x] = R + cb_blk(threadIdx.x, i, KP);
-        pb[threadIdx.x] = R + cb_blk(threadIdx.x, j, KP);
-      }
-      __syncthreads();   // also: R blocks written in the previous steps are visible
-      v4d acc = {0.0, 0.0, 0.0, 0.0};
-      acc = cb_mma_sum<true>(acc, pa, KP, pb, KP, i, q);
-#pragma unroll
-      for (int r = 0; r < 4; ++r) W[q.col() * kCbLW + q.row(r)] = sv[r] - acc[r];
-      __syncthreads();
-      double* rij = R + cb_blk(i, j, KP);
-      if (i < j) {
-        v4d o = {0.0, 0.0, 0.0, 0.0};
-        o = cb_mma<true>(o, Dinv + (long long)i * kCb * kCb, kCb, W, kCbLW, q);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) rij[(long long)q.col() * KP + q.row(r)] = o[r];
-      } else {
-        if (threadIdx.x < 64) {
-          const int f = cb_chol32(W, Di);
-          if (threadIdx.x == 0 && f != 0) {
-            fail = 1;
-            info[0] = kCb * j + f;
-          }
-        }
-        __syncthreads();
-        if (!fail) {
-          double* dj = Dinv + (long long)j * kCb * kCb;
-          for (int e = threadIdx.x; e < kCb * kCb; e += 256) {
-            const int a = e & (kCb - 1), b = e / kCb;
-            rij[(long long)b * KP + a] = W[b * kCbLW + a];
-            dj[e] = Di[b * kCbLW + a];
-          }
-        }
-      }
-      __syncthreads();
-      if (fail) return;
+    double* rjj = R + cb_blk(j, j, KP);
+    for (int e = threadIdx.x; e < kCb * kCb; e += 256) {
+      const int a = e & (kCb - 1), b = e / kCb;
+      W[b * kCbLW + a] = rjj[(long long)b * KP + a];
     }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      const int f = cb_chol32(W, Di);
+      if (threadIdx.x == 0 && f != 0) {
+        fail = 1;
+        info[0] = kCb * j + f;
+      }
+    }
+    __syncthreads();
+    if (fail) return;
+    {
+      double* dj = Dinv + (long long)j * kCb * kCb;
+      for (int e = threadIdx.x; e < kCb * kCb; e += 256) {
+        const int a = e & (kCb - 1), b = e / kCb;
+        rjj[(long long)b * KP + a] = W[b * kCbLW + a];
+        dj[e] = Di[b * kCbLW + a];
+      }
+    }
+    // panel: R_jc = Dinv_j^T A_jc, one block per wavefront at a time (Dinv_j from LDS)
+    for (int c = j + 1 + wv; c < nb; c += 4) {
+      v4d acc[2][2] = {{{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}}, {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}}};
+      double* rjc = R + cb_blk(j, c, KP);
+      cb_mma_block<true>(acc, Di, kCbLW, rjc, KP, 1.0);
+      // every lane's loads of A_jc are done before any lane overwrites it
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            rjc[(long long)(16 * tn + lm) * KP + 16 * tm + 4 * r + lk] = acc[tm][tn][r];
+    }
+    __syncthreads();
+    // trailing blocks: A_ab -= R_ja^T R_jb for j < a <= b, one block per wavefront at a time
+    const int nt = nb - 1 - j;
+    for (int q = wv; q < nt * (nt + 1) / 2; q += 4) {
+      // q -> (a, b), a <= b, row by row of the upper triangle of the trailing blocks
+      int a = 0, rem = q;
+      while (rem >= nt - a) { rem -= nt - a; ++a; }
+      const int b = a + rem;
+      const int ga = j + 1 + a, gb = j + 1 + b;
+      double* aab = R + cb_blk(ga, gb, KP);
+      v4d acc[2][2];
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) acc[tm][tn][r] = aab[(long long)(16 * tn + lm) * KP + 16 * tm + 4 * r + lk];
+      cb_mma_block<true>(acc, R + cb_blk(j, ga, KP), KP, R + cb_blk(j, gb, KP), KP, -1.0);
+#pragma unroll
+      for (int tm = 0; tm < 2; ++tm)
+#pragma unroll
+        for (int tn = 0; tn < 2; ++tn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) aab[(long long)(16 * tn + lm) * KP + 16 * tm + 4 * r + lk] = acc[tm][tn][r];
+    }
+    __syncthreads();
   }
 }
 
@@ -1261,6 +1323,9 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   if (blk) {
     // R^-1 from chol_blk_*, then Gy <- R^-T (Gy R^-1) by two GEMMs
     const int nbk = (k + kCb - 1) / kCb;
+    hipLaunchKernelGGL(chol_blk_init, dim3((kCb * nbk * kCb * nbk + 255) / 256), dim3(256), 0, c->stream, S, lds, k,
+                       c->cR);
+    PC_LAUNCH_CHECK();
     hipLaunchKernelGGL(chol_blk_factor, dim3(1), dim3(256), 0, c->stream, S, lds, k, c->cR, c->cD, c->info);
     PC_LAUNCH_CHECK();
     hipLaunchKernelGGL(chol_blk_inverse, dim3(nbk), dim3(256), 0, c->stream, c->cR, c->cD, k, c->cRi);
