@@ -288,7 +288,7 @@ def timed_only(args, atoms, dom, nev, etol):
     times = []
     for _ in range(args.steps):
         res, t = solve(prob, net=net)
-        _log(f"mixed solve: {res.iterations} iterations, {t:.3f} s")
+        _log(f"mixed solve: {res.iterations} iterations, {t:.3f} s, E {res.energies['total_ry']:.12f} Ry")
         times.append(t)
     prob.close()
     print(json.dumps({"timed_only": True, "solve_s": [round(t, 4) for t in times], "iterations": res.iterations}))

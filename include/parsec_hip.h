@@ -118,6 +118,12 @@ int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
  * 128 <= k <= 320, one workgroup up to 512, larger k uses syevd), 3 = as 2
  * with the one-workgroup reduction always, 4 = as 2 with stedc always */
 int pc_dense_set_method(void* ctx, int method);
+/* the blocked Cholesky factor and inverse of S on the context's second stream,
+ * after the work queued on the main stream; the next pc_dense_rr with this S
+ * (unchanged until then) uses it, so the factor overlaps the work queued in
+ * between.  Nothing is queued outside the blocked mode (k > 320, another
+ * triangular mode).  No counterpart in the reference. */
+int pc_dense_chol_async(void* ctx, int k, const double* S, int lds);
 /* how pc_dense_rr forms the Cholesky factor and applies R^-1: 0 = rocSOLVER
  * potrf + trtri + trmm, 1 = potrf + trsm in place, 2 = this library's blocked
  * Cholesky factor and inverse (two launches) + GEMMs (default; k > 320 uses 0).

@@ -58,6 +58,16 @@ struct DenseCtx {
   bool tri_ok;           // false after a cluster exceeded kTriMaxM (stedc from then on)
   int tri_mode;          // R^-1: 0 = trtri + trmm, 1 = triangular solves, 2 = chol_blk_* + GEMMs (k <= 320)
   int blk_fallbacks;     // attempts repeated with potrf after chol_blk_factor found a pivot that is not positive
+  // pc_dense_chol_async: the chol_blk_* kernels on a second stream (default
+  // priority), ordered by events: ev_s (S ready on the main stream) before
+  // them, ev_c after them; the next pc_dense_rr waits for ev_c
+  hipStream_t side;
+  hipEvent_t ev_s, ev_c;
+  int* cinfo;            // their potrf-style info
+  bool chol_pending;
+  int chol_k;
+  const double* chol_S;
+  int chol_lds;
   double* cR;            // chol_blk_*: factor, inverse (32 nb x 32 nb each) and the diagonal blocks' inverses
   double* cRi;
   double* cD;
@@ -95,6 +105,10 @@ __global__ void zero_info(int* info) {
   if (threadIdx.x < 4) info[threadIdx.x] = 0;
 }
 
+__global__ void copy_info(const int* src, int* dst) {
+  if (threadIdx.x == 0) dst[0] = src[0];
+}
+
 __global__ void pack_out(const int* info, double* out, int k) {
   if (threadIdx.x < 4) out[k + threadIdx.x] = (double)info[threadIdx.x];
 }
@@ -104,13 +118,13 @@ __global__ void pack_out(const int* info, double* out, int k) {
 // S = R^T R with R upper, in 32 x 32 blocks (S padded with the identity to
 // 32 nb, nb = ceil(k / 32)), then Ri = R^-1; both in column-major 32 nb x 32 nb
 // workspaces.  Replaces rocSOLVER potrf + trtri (about 30 launches at k = 275)
-// with two: chol_blk_factor (one workgroup, the blocks of R column by column:
-// R_ij = R_ii^-T (S_ij - sum_{l<i} R_li^T R_lj), the diagonal blocks factored
-// and inverted by one wavefront) and chol_blk_inverse (workgroup j: block
-// column j of R^-1 by block back substitution).  The block products run on
-// v_mfma_f64_16x16x4f64: each wavefront owns a 16 x 16 quarter of the 32 x 32
-// result (A[m][k] from lane m + 16 k, B[k][n] from lane n + 16 k, D[m][n] in
-// lane n + 16 (m % 4), register m / 4).
+// with three: chol_blk_init (the copy of S), chol_blk_factor (one workgroup,
+// right-looking by block columns: the diagonal block factored and inverted by
+// one wavefront, then R_jc = R_jj^-T A_jc and A_ab -= R_ja^T R_jb shared out
+// over the wavefronts, one 32 x 32 block each) and chol_blk_inverse
+// (workgroup j: block column j of R^-1 by block back substitution).  The block
+// products run on v_mfma_f64_16x16x4f64 (A[m][k] from lane m + 16 k, B[k][n]
+// from lane n + 16 k, D[m][n] in lane n + 16 (m % 4), register m / 4).
 constexpr int kCb = 32;            // block size
 constexpr int kCbMaxNb = 10;       // k <= 320
 constexpr int kCholMaxK = kCb * kCbMaxNb;
@@ -1222,6 +1236,14 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
   c->tri_mode = 2;
   c->blk_fallbacks = 0;
+  PC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+  PC_HIP(hipEventCreateWithFlags(&c->ev_s, hipEventDisableTiming));
+  PC_HIP(hipEventCreateWithFlags(&c->ev_c, hipEventDisableTiming));
+  PC_HIP(hipMalloc(&c->cinfo, sizeof(int)));
+  c->chol_pending = false;
+  c->chol_k = 0;
+  c->chol_S = nullptr;
+  c->chol_lds = 0;
   {
     const int kb = std::min(kmax, kCholMaxK), nbb = (kb + kCb - 1) / kCb, kp = kCb * nbb;
     PC_HIP(hipMalloc(&c->cR, (size_t)kp * kp * sizeof(double)));
@@ -1265,6 +1287,11 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
 int pc_dense_destroy(void* ctx) {
   DenseCtx* c = static_cast<DenseCtx*>(ctx);
   if (!c) return pc::PC_OK;
+  (void)hipStreamSynchronize(c->side);
+  (void)hipStreamDestroy(c->side);
+  (void)hipEventDestroy(c->ev_s);
+  (void)hipEventDestroy(c->ev_c);
+  (void)hipFree(c->cinfo);
   (void)hipFree(c->sync);
   (void)hipFree(c->pub);
   rocblas_destroy_handle(c->h);
@@ -1299,6 +1326,15 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   // positive: that attempt is repeated with rocSOLVER's potrf (the same
   // shift), which then serves the rest of the call
   bool blk = c->tri_mode == 2 && k <= kCholMaxK;
+  // a factor from pc_dense_chol_async: the main stream waits for it before it
+  // touches the factor buffers (used or not); used for the first attempt when
+  // it was made from this S
+  bool pre = false;
+  if (c->chol_pending) {
+    PC_HIP(hipStreamWaitEvent(c->stream, c->ev_c, 0));
+    pre = blk && c->chol_k == k && c->chol_S == S && c->chol_lds == lds;
+    c->chol_pending = false;
+  }
   PC_HIP(hipMemcpy2DAsync(c->S0, k * sizeof(double), S, lds * sizeof(double), k * sizeof(double), k,
                           hipMemcpyDeviceToDevice, c->stream));
   PC_HIP(hipMemcpy2DAsync(c->G0, k * sizeof(double), Gy, ldg * sizeof(double), k * sizeof(double), k,
@@ -1323,13 +1359,19 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   if (blk) {
     // R^-1 from chol_blk_*, then Gy <- R^-T (Gy R^-1) by two GEMMs
     const int nbk = (k + kCb - 1) / kCb;
-    hipLaunchKernelGGL(chol_blk_init, dim3((kCb * nbk * kCb * nbk + 255) / 256), dim3(256), 0, c->stream, S, lds, k,
-                       c->cR);
-    PC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(chol_blk_factor, dim3(1), dim3(256), 0, c->stream, S, lds, k, c->cR, c->cD, c->info);
-    PC_LAUNCH_CHECK();
-    hipLaunchKernelGGL(chol_blk_inverse, dim3(nbk), dim3(256), 0, c->stream, c->cR, c->cD, k, c->cRi);
-    PC_LAUNCH_CHECK();
+    if (pre) {
+      hipLaunchKernelGGL(copy_info, dim3(1), dim3(64), 0, c->stream, c->cinfo, c->info);
+      PC_LAUNCH_CHECK();
+      pre = false;   // a retry factors again on this stream
+    } else {
+      hipLaunchKernelGGL(chol_blk_init, dim3((kCb * nbk * kCb * nbk + 255) / 256), dim3(256), 0, c->stream, S, lds,
+                         k, c->cR);
+      PC_LAUNCH_CHECK();
+      hipLaunchKernelGGL(chol_blk_factor, dim3(1), dim3(256), 0, c->stream, S, lds, k, c->cR, c->cD, c->info);
+      PC_LAUNCH_CHECK();
+      hipLaunchKernelGGL(chol_blk_inverse, dim3(nbk), dim3(256), 0, c->stream, c->cR, c->cD, k, c->cRi);
+      PC_LAUNCH_CHECK();
+    }
     const double zero = 0.0;
     PC_RB(rocblas_dgemm(c->h, rocblas_operation_none, rocblas_operation_none, k, k, k, &one, Gy, ldg, c->cRi,
                         kCb * nbk, &zero, c->tmp, k));
@@ -1477,6 +1519,35 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   }
   if (lam_host)
     for (int i = 0; i < k; ++i) lam_host[i] = c->host_out[i];
+  return pc::PC_OK;
+}
+
+// The blocked Cholesky factor and inverse of S (k x k, ld lds) on the
+// context's second stream, after the work already queued on the main stream
+// (which produces S): the next pc_dense_rr with this S uses it, so the factor
+// overlaps what the caller queues in between (H Y and its Gram matrix).  S
+// must stay unchanged until that call.  Without the blocked kernels (another
+// triangular mode, k > 320) nothing is queued.
+int pc_dense_chol_async(void* ctx, int k, const double* S, int lds) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c && S, "null pointer");
+  PC_CHECK_ARG(k >= 1 && k <= c->kmax && lds >= k, "bad size");
+  if (c->tri_mode != 2 || k > kCholMaxK) return pc::PC_OK;
+  if (c->chol_pending) PC_HIP(hipStreamWaitEvent(c->stream, c->ev_c, 0));   // one factor in flight at a time
+  PC_HIP(hipEventRecord(c->ev_s, c->stream));
+  PC_HIP(hipStreamWaitEvent(c->side, c->ev_s, 0));
+  const int nbk = (k + kCb - 1) / kCb;
+  hipLaunchKernelGGL(chol_blk_init, dim3((kCb * nbk * kCb * nbk + 255) / 256), dim3(256), 0, c->side, S, lds, k, c->cR);
+  PC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(chol_blk_factor, dim3(1), dim3(256), 0, c->side, S, lds, k, c->cR, c->cD, c->cinfo);
+  PC_LAUNCH_CHECK();
+  hipLaunchKernelGGL(chol_blk_inverse, dim3(nbk), dim3(256), 0, c->side, c->cR, c->cD, k, c->cRi);
+  PC_LAUNCH_CHECK();
+  PC_HIP(hipEventRecord(c->ev_c, c->side));
+  c->chol_pending = true;
+  c->chol_k = k;
+  c->chol_S = S;
+  c->chol_lds = lds;
   return pc::PC_OK;
 }
 

@@ -197,10 +197,12 @@ class Eigensolver:
     # -------------------------------------------------------------- Rayleigh-Ritz
     def rayleigh_ritz(self, Y: torch.Tensor, k: int):
         HY = self._other(Y)
-        self.H.apply(Y, HY, k)
         S = self.S.view(-1)[:k * k]
         G = self.G.view(-1)[:k * k]
         self.la.gram(Y, Y, k, k, S, sym=True)
+        # the Cholesky factor of S runs on a second stream while H Y and G are formed
+        self.la.dense_chol_async(S, k)
+        self.H.apply(Y, HY, k)
         # G = Y^T (H Y) is symmetric: only the upper tiles are computed and mirrored
         # (the dense stage symmetrises G again after the Cholesky transform)
         self.la.gram(Y, HY, k, k, G, sym=True)

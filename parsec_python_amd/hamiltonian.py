@@ -138,6 +138,11 @@ class BlockLinalg:
         _lib.call(f"pc_rotate_{suffix}", ptr(X), X.shape[1], ptr(Q), kx, ptr(out), out.shape[1], self.n, kx, ky,
                   ptr(work), nbytes.value, stream())
 
+    def dense_chol_async(self, S: torch.Tensor, k: int) -> None:
+        """Queue the Cholesky factor of S (k x k) on the dense stage's second
+        stream; the next dense_rr with this S uses it (S must stay unchanged)."""
+        _lib.call("pc_dense_chol_async", self.dense, k, ptr(S), k)
+
     def dense_rr(self, S: torch.Tensor, G: torch.Tensor, k: int):
         """Return (T (k x k column-major FP64 device), ritz values (host numpy))."""
         import numpy as np

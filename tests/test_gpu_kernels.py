@@ -414,6 +414,38 @@ def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
 
 
 @pytest.mark.parametrize("k", [34, 275])
+def test_dense_rr_with_cholesky_on_the_second_stream(gpu, k):
+    """The Cholesky factor queued on the dense stage's second stream
+    (pc_dense_chol_async) while other work runs on the main stream gives
+    pc_dense_rr the same bits as computing it in place; a factor queued for a
+    different S is waited for and not used."""
+    import torch
+    from parsec_python_amd.hamiltonian import BlockLinalg
+    rng = np.random.default_rng(11 + k)
+    B = rng.standard_normal((2 * k, k))
+    S = B.T @ B / B.shape[0] + 0.5 * np.eye(k)
+    M = rng.standard_normal((k, k))
+    G = M + M.T
+    S0 = torch.tensor(S.T.copy(), device="cuda").view(-1)
+    G0 = torch.tensor(G.T.copy(), device="cuda").view(-1)
+    la = BlockLinalg(10, k)
+    T1, lam1 = la.dense_rr(S0.clone(), G0.clone(), k)
+    T1 = T1.clone()
+    Sd, Gd = S0.clone(), G0.clone()
+    la.dense_chol_async(Sd, k)
+    busy = torch.randn(4096, 4096, device="cuda")   # main-stream work in between
+    busy = busy @ busy
+    T2, lam2 = la.dense_rr(Sd, Gd, k)
+    assert torch.equal(T1, T2) and np.array_equal(lam1, lam2)
+    other = S0 * 2.0                                 # kept alive: a different S at another address
+    la.dense_chol_async(other, k)                    # not the S below: waited for, not used
+    T3, lam3 = la.dense_rr(S0.clone(), G0.clone(), k)
+    assert torch.equal(T1, T3) and np.array_equal(lam1, lam3)
+    torch.cuda.synchronize()
+    la.close()
+
+
+@pytest.mark.parametrize("k", [34, 275])
 def test_dense_rr_after_nan_filled_device_memory(gpu, k):
     """Device memory released by earlier work and handed to a new dense
     context holds whatever was there (here NaN): the Rayleigh-Ritz stage must
