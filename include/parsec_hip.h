@@ -124,6 +124,13 @@ int pc_dense_set_method(void* ctx, int method);
  * between.  Nothing is queued outside the blocked mode (k > 320, another
  * triangular mode).  No counterpart in the reference. */
 int pc_dense_chol_async(void* ctx, int k, const double* S, int lds);
+/* S = Y^T Y (pc_gram_f32 / pc_gram_f64, sym) and then the blocked Cholesky
+ * factor and inverse, both on the context's second stream after the work
+ * queued on the main stream; the next pc_dense_rr waits for them and uses the
+ * factor.  work: a Gram workspace of its own (pc_gram_workspace); Y, S and
+ * work unchanged until then.  No counterpart in the reference. */
+int pc_dense_gram_chol_async(void* ctx, int fp32, const void* Y, long long ldy, long long n, int k, double* S,
+                             int lds, void* work, size_t work_bytes);
 /* how pc_dense_rr forms the Cholesky factor and applies R^-1: 0 = rocSOLVER
  * potrf + trtri + trmm, 1 = potrf + trsm in place, 2 = this library's blocked
  * Cholesky factor and inverse (two launches) + GEMMs (default; k > 320 uses 0).

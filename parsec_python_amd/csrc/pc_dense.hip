@@ -14,6 +14,13 @@
 #include <type_traits>
 #include <vector>
 
+extern "C" {
+int pc_gram_f64(const double* X, long long ldx, const double* Y, long long ldy, long long n, int kx, int ky,
+                double* S, long long lds, int sym, void* work, size_t work_bytes, void* stream);
+int pc_gram_f32(const float* X, long long ldx, const float* Y, long long ldy, long long n, int kx, int ky,
+                double* S, long long lds, int sym, void* work, size_t work_bytes, void* stream);
+}
+
 namespace {
 
 constexpr int kTriMaxM = 32;
@@ -65,6 +72,7 @@ struct DenseCtx {
   hipEvent_t ev_s, ev_c;
   int* cinfo;            // their potrf-style info
   bool chol_pending;
+  bool chol_made;        // the pending side work includes the factor (else only S)
   int chol_k;
   const double* chol_S;
   int chol_lds;
@@ -1241,6 +1249,7 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipEventCreateWithFlags(&c->ev_c, hipEventDisableTiming));
   PC_HIP(hipMalloc(&c->cinfo, sizeof(int)));
   c->chol_pending = false;
+  c->chol_made = false;
   c->chol_k = 0;
   c->chol_S = nullptr;
   c->chol_lds = 0;
@@ -1332,7 +1341,7 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
   bool pre = false;
   if (c->chol_pending) {
     PC_HIP(hipStreamWaitEvent(c->stream, c->ev_c, 0));
-    pre = blk && c->chol_k == k && c->chol_S == S && c->chol_lds == lds;
+    pre = blk && c->chol_made && c->chol_k == k && c->chol_S == S && c->chol_lds == lds;
     c->chol_pending = false;
   }
   PC_HIP(hipMemcpy2DAsync(c->S0, k * sizeof(double), S, lds * sizeof(double), k * sizeof(double), k,
@@ -1528,27 +1537,62 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
 // overlaps what the caller queues in between (H Y and its Gram matrix).  S
 // must stay unchanged until that call.  Without the blocked kernels (another
 // triangular mode, k > 320) nothing is queued.
-int pc_dense_chol_async(void* ctx, int k, const double* S, int lds) {
-  DenseCtx* c = static_cast<DenseCtx*>(ctx);
-  PC_CHECK_ARG(c && S, "null pointer");
-  PC_CHECK_ARG(k >= 1 && k <= c->kmax && lds >= k, "bad size");
-  if (c->tri_mode != 2 || k > kCholMaxK) return pc::PC_OK;
-  if (c->chol_pending) PC_HIP(hipStreamWaitEvent(c->stream, c->ev_c, 0));   // one factor in flight at a time
+static int side_begin(DenseCtx* c) {
+  if (c->chol_pending) PC_HIP(hipStreamWaitEvent(c->stream, c->ev_c, 0));   // one batch of side work at a time
   PC_HIP(hipEventRecord(c->ev_s, c->stream));
   PC_HIP(hipStreamWaitEvent(c->side, c->ev_s, 0));
-  const int nbk = (k + kCb - 1) / kCb;
-  hipLaunchKernelGGL(chol_blk_init, dim3((kCb * nbk * kCb * nbk + 255) / 256), dim3(256), 0, c->side, S, lds, k, c->cR);
-  PC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(chol_blk_factor, dim3(1), dim3(256), 0, c->side, S, lds, k, c->cR, c->cD, c->cinfo);
-  PC_LAUNCH_CHECK();
-  hipLaunchKernelGGL(chol_blk_inverse, dim3(nbk), dim3(256), 0, c->side, c->cR, c->cD, k, c->cRi);
-  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+static int side_end(DenseCtx* c, int k, const double* S, int lds) {
+  c->chol_made = false;
+  if (c->tri_mode == 2 && k <= kCholMaxK) {
+    const int nbk = (k + kCb - 1) / kCb;
+    hipLaunchKernelGGL(chol_blk_init, dim3((kCb * nbk * kCb * nbk + 255) / 256), dim3(256), 0, c->side, S, lds, k,
+                       c->cR);
+    PC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(chol_blk_factor, dim3(1), dim3(256), 0, c->side, S, lds, k, c->cR, c->cD, c->cinfo);
+    PC_LAUNCH_CHECK();
+    hipLaunchKernelGGL(chol_blk_inverse, dim3(nbk), dim3(256), 0, c->side, c->cR, c->cD, k, c->cRi);
+    PC_LAUNCH_CHECK();
+    c->chol_made = true;
+  }
   PC_HIP(hipEventRecord(c->ev_c, c->side));
   c->chol_pending = true;
   c->chol_k = k;
   c->chol_S = S;
   c->chol_lds = lds;
   return pc::PC_OK;
+}
+
+int pc_dense_chol_async(void* ctx, int k, const double* S, int lds) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c && S, "null pointer");
+  PC_CHECK_ARG(k >= 1 && k <= c->kmax && lds >= k, "bad size");
+  if (c->tri_mode != 2 || k > kCholMaxK) return pc::PC_OK;
+  if (side_begin(c) != pc::PC_OK) return pc::PC_ERR_HIP;
+  return side_end(c, k, S, lds);
+}
+
+// S = Y^T Y (symmetric Gram matrix of the k columns of Y, FP32 when fp32 = 1,
+// else FP64; n points, ld ldy) and then its blocked Cholesky factor and
+// inverse, both on the context's second stream after the work queued on the
+// main stream (which produces Y): the main stream can form H Y and its Gram
+// matrix meanwhile.  work (work_bytes from pc_gram_workspace) must not be
+// the workspace of the main stream's Gram matrix.  Y, S and work must stay
+// unchanged until the next pc_dense_rr, which waits for this work.
+int pc_dense_gram_chol_async(void* ctx, int fp32, const void* Y, long long ldy, long long n, int k, double* S,
+                             int lds, void* work, size_t work_bytes) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c && Y && S && work, "null pointer");
+  PC_CHECK_ARG(k >= 1 && k <= c->kmax && lds >= k, "bad size");
+  if (side_begin(c) != pc::PC_OK) return pc::PC_ERR_HIP;
+  const int st = fp32 ? pc_gram_f32(static_cast<const float*>(Y), ldy, static_cast<const float*>(Y), ldy, n, k, k, S,
+                                    lds, 1, work, work_bytes, c->side)
+                      : pc_gram_f64(static_cast<const double*>(Y), ldy, static_cast<const double*>(Y), ldy, n, k, k,
+                                    S, lds, 1, work, work_bytes, c->side);
+  if (st != pc::PC_OK) return st;
+  return side_end(c, k, S, lds);
 }
 
 // Eigensolver of the projected k x k problem: 0 = divide and conquer

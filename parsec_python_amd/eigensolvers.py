@@ -59,8 +59,11 @@ def rand_block(out: torch.Tensor, n: int, k: int, seed: int, normal: bool) -> No
 
 
 class Eigensolver:
-    def __init__(self, H: Hamiltonian, kmax: int, seed: int = 0):
+    def __init__(self, H: Hamiltonian, kmax: int, seed: int = 0, overlap: bool = True):
         self.H = H
+        # Rayleigh-Ritz: S = Y^T Y and its Cholesky factor on a second stream
+        # while H Y and G = Y^T H Y are formed on the main one
+        self.overlap = overlap
         self.n = H.n
         self.ld = H.ld
         self.kmax = kmax
@@ -199,9 +202,11 @@ class Eigensolver:
         HY = self._other(Y)
         S = self.S.view(-1)[:k * k]
         G = self.G.view(-1)[:k * k]
-        self.la.gram(Y, Y, k, k, S, sym=True)
-        # the Cholesky factor of S runs on a second stream while H Y and G are formed
-        self.la.dense_chol_async(S, k)
+        if self.overlap:
+            # S and its Cholesky factor on a second stream while H Y and G are formed
+            self.la.gram_chol_async(Y, k, S)
+        else:
+            self.la.gram(Y, Y, k, k, S, sym=True)
         self.H.apply(Y, HY, k)
         # G = Y^T (H Y) is symmetric: only the upper tiles are computed and mirrored
         # (the dense stage symmetrises G again after the Cholesky transform)

@@ -143,6 +143,16 @@ class BlockLinalg:
         stream; the next dense_rr with this S uses it (S must stay unchanged)."""
         _lib.call("pc_dense_chol_async", self.dense, k, ptr(S), k)
 
+    def gram_chol_async(self, Y: torch.Tensor, k: int, S: torch.Tensor) -> None:
+        """S = Y^T Y (k columns) and its Cholesky factor on the dense stage's
+        second stream; the next dense_rr with this S waits for them and uses
+        the factor (Y and S must stay unchanged until then)."""
+        nbytes = C.c_size_t()
+        _lib.call("pc_gram_workspace", self.n, k, k, 1, C.byref(nbytes))
+        work = self.scratch.bytes("gram_side", nbytes.value)
+        _lib.call("pc_dense_gram_chol_async", self.dense, int(Y.dtype != F64), ptr(Y), Y.shape[1], self.n, k, ptr(S),
+                  k, ptr(work), nbytes.value)
+
     def dense_rr(self, S: torch.Tensor, G: torch.Tensor, k: int):
         """Return (T (k x k column-major FP64 device), ritz values (host numpy))."""
         import numpy as np

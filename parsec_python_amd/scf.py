@@ -82,6 +82,9 @@ class SCFConfig:
     # library's two-launch blocked factor and inverse, then GEMMs; k <= 320,
     # larger k uses "rocsolver") or "rocsolver" (potrf + trtri + trmm)
     dense_cholesky: str = "blocked"
+    # Rayleigh-Ritz: S = Y^T Y and its Cholesky factor on a second stream
+    # while H Y and G are formed (the same bits either way)
+    rr_overlap: bool = True
     # extra fixed-potential filter passes after first_filt when the SCF starts
     # from a given (learned) density, until the mean occupied Ritz value moves
     # by less than start_warmup_tol Ha (see run_scf).  Off by default: on
@@ -224,7 +227,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     occ2_dev = torch.empty(k, **dev)
 
     H.dtype = prob.dtype
-    eig = Eigensolver(H, k, seed=cfg.seed)
+    eig = Eigensolver(H, k, seed=cfg.seed, overlap=cfg.rr_overlap)
     dense_method = {"syevd": 0, "syevj": 1, "tridiag": 2}[cfg.dense_method]
     tri_mode = {"rocsolver": 0, "blocked": 2}[cfg.dense_cholesky]
     _lib.call("pc_dense_set_method", eig.la.dense, dense_method)
@@ -329,7 +332,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
                                               (cfg.polish_err is not None and err < cfg.polish_err)):
             # finish in FP64: convert the current Ritz block and keep iterating
             H.dtype = F64
-            eig64 = Eigensolver(H, k, seed=cfg.seed)
+            eig64 = Eigensolver(H, k, seed=cfg.seed, overlap=cfg.rr_overlap)
             _lib.call("pc_dense_set_method", eig64.la.dense, dense_method)
             _lib.call("pc_dense_set_triangular", eig64.la.dense, tri_mode)
             eig64.seeds = eig.seeds

@@ -54,3 +54,25 @@ def test_scf_mixed_precision_matches_fp64(gpu):
     assert abs(e64 - e32) < 1e-6
     occ = out["fp64"].occupations > 0.5
     assert np.max(np.abs(out["fp64"].eigenvalues[:nev][occ] - out["mixed"].eigenvalues[:nev][occ])) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_scf_overlapped_rayleigh_ritz_gives_the_same_bits(gpu, precision):
+    """S = Y^T Y and its Cholesky factor on the dense stage's second stream
+    (while H Y and G are formed on the main one) against the same steps in
+    sequence on one stream: the same SCF to the last bit."""
+    import torch
+    from parsec_python_amd.scf import SCFConfig, SCFProblem, run_scf
+    atoms, dom, nev = _small_h2o()
+    out = {}
+    for overlap in (False, True):
+        cfg = SCFConfig(tol=1e-7, maxits=40, domain="sphere", seed=3, precision=precision, rr_overlap=overlap)
+        prob = SCFProblem(atoms, dom, nev, cfg)
+        res = run_scf(prob)
+        out[overlap] = (res.iterations, res.energies["total_ry"], res.rho.clone(), np.array(res.eigenvalues))
+        prob.close()
+    print(f"  {precision}: iterations {out[False][0]} / {out[True][0]}", flush=True)
+    assert out[False][0] == out[True][0]
+    assert out[False][1] == out[True][1]
+    assert torch.equal(out[False][2], out[True][2])
+    assert np.array_equal(out[False][3], out[True][3])
