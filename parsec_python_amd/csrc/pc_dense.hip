@@ -28,12 +28,7 @@ constexpr int kTriClusterFlag = -7;
 constexpr int kTriMaxK = 512;
 
 struct TriWs {
-  double* u1;   // [k*k] U diagonal
-  double* u2;   // [k*k] U first superdiagonal
-  double* u3;   // [k*k] U second superdiagonal (fill-in of pivoting)
-  double* l;    // [k*k] multipliers
-  double* x;    // [k*k] eigenvector iterates
-  unsigned char* piv;   // [k*k]
+  double* x;    // [k*k] eigenvectors of T, element j of vector i at [j * k + i]
   double* lam;  // [k] bisection eigenvalues
   double* par;  // [4] gl, gu, ||T||, pivmin
 };
@@ -850,8 +845,9 @@ __global__ __launch_bounds__(256) void sytrd_lower_mwg(const double* __restrict_
 //                Gram-Schmidt, twice) and rotated to the eigenvectors of their
 //                projected m x m matrix (Jacobi), so vectors of close or equal
 //                eigenvalues come out orthogonal and exact within the cluster.
-// Per-thread arrays live in a workspace interleaved as [j * k + i] (element j
-// of eigenvalue i), so the k threads' accesses are contiguous.  A cluster of
+// The eigenvectors pass from tri_invit to tri_cluster through a workspace
+// interleaved as [j * k + i] (element j of eigenvalue i), so the threads'
+// accesses are contiguous; tri_invit's factors stay in its LDS.  A cluster of
 // more than kTriMaxM eigenvalues is reported in info (kTriClusterFlag) and the
 // caller repeats the step with rocSOLVER's stedc.
 // 1/q to about one ulp: the hardware reciprocal and two Newton steps (a
@@ -937,23 +933,56 @@ __global__ __launch_bounds__(256) void tri_bisect(const double* __restrict__ d, 
   if (lane == 0) ws.lam[i] = 0.5 * (lo + hi);
 }
 
-// 32 threads per workgroup, each workgroup on its own CU: the iterate x of
-// each thread sits in LDS ([j][thread], 8 B x 32 per row, up to 128 KB) and
-// the factors in the interleaved workspace, read in blocks of TB rows whose
-// loads go out before the block's dependent chain.
-constexpr int kInvT = 32;
-__global__ __launch_bounds__(kInvT) void tri_invit(const double* __restrict__ d, const double* __restrict__ e, int k,
-                                                   TriWs ws) {
+// Inverse iteration, one thread per eigenvalue.  Each thread's LU factors of
+// T - sigma I (the reciprocal of U's diagonal, U's first superdiagonal, the
+// multipliers and the pivot choices; U's second superdiagonal is nonzero only
+// after a row swap and then equals e[j+1]) and its iterate sit in dynamic LDS,
+// interleaved over the workgroup's TI threads ([row][thread], 33 bytes per row
+// and thread): TI = 16 where 16 threads' rows fit in kInvLdsBytes (k <= 290),
+// else 8.  TI is a compile-time constant so every row's LDS address is a base
+// register plus an immediate offset; each pass walks the rows in blocks of
+// kInvTB whose LDS reads are issued together before the block's dependent
+// chain, with a row-by-row tail, and the pivot choices are selects, not
+// branches (the threads of a wavefront pivot differently).  The start vector
+// is generated inside the first forward solve.  At k = 275 this is 155 us
+// against 255 us for the same arithmetic with a run-time thread count and a
+// bounds check per row (and 270 us with the factors in global memory): the
+// passes are bound by instruction issue and FP64 latency of one wave per CU.
+constexpr int kInvLdsBytes = 150 * 1024;
+constexpr int kInvTB = 16;
+inline int tri_invit_threads(int k) { return (size_t)33 * 16 * k <= (size_t)kInvLdsBytes ? 16 : 8; }
+inline size_t tri_invit_lds(int k) { return (size_t)33 * k * tri_invit_threads(k); }
+
+// fixed pseudo-random start value of element j of vector i, in [-0.5, 0.5)
+// (no two eigenvalues start alike)
+__device__ __forceinline__ double tri_start(int j, int i) {
+  unsigned h = (unsigned)j * 0x9E3779B1u ^ (unsigned)(i + 1) * 0x85EBCA77u;
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return (double)(h & 0xFFFFFFu) * (1.0 / 16777216.0) - 0.5;
+}
+
+template <int TI>
+__global__ __launch_bounds__(TI) void tri_invit(const double* __restrict__ d, const double* __restrict__ e, int k,
+                                                TriWs ws) {
+  constexpr int TB = kInvTB;
   __shared__ double sdd[kTriMaxK], sde[kTriMaxK];
-  __shared__ double sx[kTriMaxK * kInvT];
-  for (int j = threadIdx.x; j < k; j += kInvT) {
+  extern __shared__ double dyn[];
+  for (int j = threadIdx.x; j < k; j += TI) {
     sdd[j] = d[j];
-    sde[j] = j + 1 < k ? e[j] : 0.0;
+    sde[j] = j + 1 < k ? e[j] : 0.0;   // sde[k - 1] = 0
   }
   __syncthreads();
   const int lt = threadIdx.x;
-  const int i = blockIdx.x * kInvT + lt;
+  const int i = blockIdx.x * TI + lt;
   if (i >= k) return;
+  // row j of thread lt at [j * TI] of each array
+  double* RP = dyn + lt;                                   // 1 / U(j,j), the pivot held away from 0
+  double* U2 = dyn + (size_t)k * TI + lt;                  // U(j,j+1)
+  double* LM = dyn + (size_t)2 * k * TI + lt;              // multiplier of step j
+  double* XV = dyn + (size_t)3 * k * TI + lt;              // iterate
+  unsigned char* PV = reinterpret_cast<unsigned char*>(dyn + (size_t)4 * k * TI) + lt;   // rows j, j+1 swapped
   const double eps = 2.220446049250313e-16;
   const double tn = ws.par[2];
   const double tiny = eps * tn;
@@ -962,98 +991,93 @@ __global__ __launch_bounds__(kInvT) void tri_invit(const double* __restrict__ d,
   int rep = 0;
   for (int t = i; t > 0 && ws.lam[t] - ws.lam[t - 1] < delta; --t) ++rep;
   const double sg = ws.lam[i] + rep * delta;
-  auto at = [&](int j) { return (long long)j * k + i; };
-  auto X = [&](int j) -> double& { return sx[j * kInvT + lt]; };
-  // LU with partial pivoting of T - sg I; row j of U: u1 (j,j), u2 (j,j+1), u3 (j,j+2)
+  auto rcp_piv = [&](double u) { return tri_rcp(fabs(u) > tiny ? u : (u >= 0.0 ? tiny : -tiny)); };
+  // LU with partial pivoting of T - sg I: cd, cu are the pending row's (j,j), (j,j+1)
   double cd = sdd[0] - sg, cu = k > 1 ? sde[0] : 0.0;
-  for (int j = 0; j + 1 < k; ++j) {
-    const double sub = sde[j], nd = sdd[j + 1] - sg, nu = j + 2 < k ? sde[j + 1] : 0.0;
-    if (fabs(cd) >= fabs(sub)) {
-      const double m = cd != 0.0 ? sub * tri_rcp(cd) : 0.0;
-      ws.u1[at(j)] = cd; ws.u2[at(j)] = cu; ws.u3[at(j)] = 0.0; ws.l[at(j)] = m; ws.piv[at(j)] = 0;
-      cd = nd - m * cu;
-      cu = nu;
-    } else {
-      const double m = cd * tri_rcp(sub);
-      ws.u1[at(j)] = sub; ws.u2[at(j)] = nd; ws.u3[at(j)] = nu; ws.l[at(j)] = m; ws.piv[at(j)] = 1;
-      cd = cu - m * nd;
-      cu = -m * nu;
+  auto lu_step = [&](int j, double sub, double dnext, double nu) {
+    const double nd = dnext - sg;
+    const bool pv = fabs(cd) < fabs(sub);
+    const double den = pv ? sub : cd, num = pv ? cd : sub;
+    const double m = den != 0.0 ? num * tri_rcp(den) : 0.0;
+    RP[j * TI] = rcp_piv(den);
+    U2[j * TI] = pv ? nd : cu;
+    LM[j * TI] = m;
+    PV[j * TI] = pv;
+    const double a = pv ? cu : nd, b = pv ? nd : cu;
+    cd = a - m * b;
+    cu = pv ? -m * nu : nu;
+  };
+  int j0 = 0;
+  for (; j0 + TB <= k - 1; j0 += TB) {
+    double rs[TB], rn[TB], ru[TB];
+#pragma unroll
+    for (int t = 0; t < TB; ++t) {
+      rs[t] = sde[j0 + t];
+      rn[t] = sdd[j0 + t + 1];
+      ru[t] = sde[j0 + t + 1];
     }
+#pragma unroll
+    for (int t = 0; t < TB; ++t) lu_step(j0 + t, rs[t], rn[t], ru[t]);
   }
-  // the last row has no superdiagonals; the back substitution still reads them
-  // (times zero), so they must be zeros, not whatever the workspace held
-  ws.u1[at(k - 1)] = cd;
-  ws.u2[at(k - 1)] = 0.0;
-  ws.u3[at(k - 1)] = 0.0;
-  // fixed pseudo-random start vector in [-0.5, 0.5) (no two eigenvalues start alike)
-  for (int j = 0; j < k; ++j) {
-    unsigned h = (unsigned)j * 0x9E3779B1u ^ (unsigned)(i + 1) * 0x85EBCA77u;
-    h ^= h >> 15;
-    h *= 0x2C1B3C6Du;
-    h ^= h >> 12;
-    X(j) = (double)(h & 0xFFFFFFu) * (1.0 / 16777216.0) - 0.5;
-  }
-  constexpr int TB = 16;
+  for (int j = j0; j < k - 1; ++j) lu_step(j, sde[j], sdd[j + 1], sde[j + 1]);
+  // the last row has no superdiagonals
+  RP[(k - 1) * TI] = rcp_piv(cd);
+  U2[(k - 1) * TI] = 0.0;
+  PV[(k - 1) * TI] = 0;
   double sc = 1.0;
   for (int it = 0; it < 3; ++it) {
-    double yc = X(0);
-    for (int j0 = 0; j0 + 1 < k; j0 += TB) {
-      double lb[TB];
-      unsigned char pb[TB];
+    const bool first = it == 0;
+    // L y = P x (forward, the row swaps as recorded)
+    double yc = first ? tri_start(0, i) : XV[0];
+    auto fw_step = [&](int j, double m, double yn, bool p) {
+      XV[j * TI] = p ? yn : yc;
+      const double a = p ? yc : yn, b = p ? yn : yc;
+      yc = a - m * b;
+    };
+    j0 = 0;
+    for (; j0 + TB <= k - 1; j0 += TB) {
+      double rm[TB], rx[TB];
+      bool rp[TB];
 #pragma unroll
       for (int t = 0; t < TB; ++t) {
-        const int j = j0 + t;
-        if (j + 1 < k) {
-          lb[t] = ws.l[at(j)];
-          pb[t] = ws.piv[at(j)];
-        }
+        rm[t] = LM[(j0 + t) * TI];
+        rx[t] = first ? tri_start(j0 + t + 1, i) : XV[(j0 + t + 1) * TI];
+        rp[t] = PV[(j0 + t) * TI];
       }
 #pragma unroll
-      for (int t = 0; t < TB; ++t) {
-        const int j = j0 + t;
-        if (j + 1 < k) {
-          const double yn = X(j + 1);
-          if (pb[t]) {
-            X(j) = yn;
-            yc = yc - lb[t] * yn;
-          } else {
-            X(j) = yc;
-            yc = yn - lb[t] * yc;
-          }
-        }
-      }
+      for (int t = 0; t < TB; ++t) fw_step(j0 + t, rm[t], rx[t], rp[t]);
     }
-    X(k - 1) = yc;
+    for (int j = j0; j < k - 1; ++j) fw_step(j, LM[j * TI], first ? tri_start(j + 1, i) : XV[(j + 1) * TI], PV[j * TI]);
+    XV[(k - 1) * TI] = yc;
+    // U x = y (backward); the previous pass's normalisation is folded into this one
     double y1 = 0.0, y2 = 0.0, nrm = 0.0;
-    for (int j1 = k - 1; j1 >= 0; j1 -= TB) {
-      double ub1[TB], ub2[TB], ub3[TB];
+    auto bw_step = [&](int j, double rp, double u2, double u3, double x) {
+      const double y = (sc * x - u2 * y1 - u3 * y2) * rp;
+      XV[j * TI] = y;
+      y2 = y1;
+      y1 = y;
+      nrm += y * y;
+    };
+    // U(j,j+2) = sde[j + 1] after a swap (row k - 1 never swaps, and sde[k - 1] = 0)
+    int jt = k - 1;
+    for (; jt - TB + 1 >= 0; jt -= TB) {
+      double r1[TB], r2[TB], r3[TB], rx[TB];
 #pragma unroll
       for (int t = 0; t < TB; ++t) {
-        const int j = j1 - t;
-        if (j >= 0) {
-          ub1[t] = ws.u1[at(j)];
-          ub2[t] = ws.u2[at(j)];
-          ub3[t] = ws.u3[at(j)];
-        }
+        const int j = jt - t;
+        r1[t] = RP[j * TI];
+        r2[t] = U2[j * TI];
+        r3[t] = PV[j * TI] ? sde[min(j + 1, k - 1)] : 0.0;
+        rx[t] = XV[j * TI];
       }
 #pragma unroll
-      for (int t = 0; t < TB; ++t) {
-        const int j = j1 - t;
-        if (j >= 0) {
-          const double u = ub1[t];
-          const double p = fabs(u) > tiny ? u : (u >= 0.0 ? tiny : -tiny);
-          // the previous pass's normalisation is folded into this one
-          const double y = (sc * X(j) - ub2[t] * y1 - ub3[t] * y2) * tri_rcp(p);
-          X(j) = y;
-          y2 = y1;
-          y1 = y;
-          nrm += y * y;
-        }
-      }
+      for (int t = 0; t < TB; ++t) bw_step(jt - t, r1[t], r2[t], r3[t], rx[t]);
     }
+    for (int j = jt; j >= 0; --j)
+      bw_step(j, RP[j * TI], U2[j * TI], PV[j * TI] ? sde[min(j + 1, k - 1)] : 0.0, XV[j * TI]);
     sc = 1.0 / sqrt(nrm);
   }
-  for (int j = 0; j < k; ++j) ws.x[at(j)] = sc * X(j);
+  for (int j = 0; j < k; ++j) ws.x[(long long)j * k + i] = sc * XV[j * TI];
 }
 
 __device__ __forceinline__ double tri_block_sum(double v, double* red) {
@@ -1261,23 +1285,24 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   }
   c->tri_ok = kmax <= kTriMaxK;
   if (c->tri_ok) {
-    // tri_cluster keeps a cluster's vectors in dynamic LDS: kTriMaxM * kmax doubles
-    const int lds_bytes = (int)((size_t)kTriMaxM * kmax * sizeof(double));
+    // dynamic LDS limits of the tridiagonal kernels, for every k they take (a
+    // process-wide attribute: set to the largest need, not this context's):
+    // tri_cluster a cluster's vectors, kTriMaxM * k doubles; tri_invit its
+    // factors and iterates, at most kInvLdsBytes
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(&tri_cluster), hipFuncAttributeMaxDynamicSharedMemorySize,
-                            lds_bytes) != hipSuccess)
+                            (int)((size_t)kTriMaxM * kTriMaxK * sizeof(double))) != hipSuccess)
+      c->tri_ok = false;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&tri_invit<16>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kInvLdsBytes) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&tri_invit<8>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, kInvLdsBytes) != hipSuccess)
       c->tri_ok = false;
   }
   c->tri_fallbacks = 0;
   c->tri_lam = nullptr;
   c->tw = TriWs{};
   if (c->tri_ok) {
-    const size_t kk = (size_t)kmax * kmax;
-    PC_HIP(hipMalloc(&c->tw.u1, 5 * kk * sizeof(double)));
-    c->tw.u2 = c->tw.u1 + kk;
-    c->tw.u3 = c->tw.u2 + kk;
-    c->tw.l = c->tw.u3 + kk;
-    c->tw.x = c->tw.l + kk;
-    PC_HIP(hipMalloc(&c->tw.piv, kk));
+    PC_HIP(hipMalloc(&c->tw.x, (size_t)kmax * kmax * sizeof(double)));
     PC_HIP(hipMalloc(&c->tw.lam, (2 * (size_t)kmax + 4) * sizeof(double)));
     c->tri_lam = c->tw.lam + kmax;
     c->tw.par = c->tri_lam + kmax;
@@ -1308,7 +1333,7 @@ int pc_dense_destroy(void* ctx) {
   (void)hipFree(c->resid); (void)hipFree(c->sweeps); (void)hipFree(c->tau); (void)hipFree(c->P);
   (void)hipFree(c->S0); (void)hipFree(c->G0);
   (void)hipFree(c->cR); (void)hipFree(c->cRi); (void)hipFree(c->cD);
-  if (c->tw.u1) { (void)hipFree(c->tw.u1); (void)hipFree(c->tw.piv); (void)hipFree(c->tw.lam); }
+  if (c->tw.x) { (void)hipFree(c->tw.x); (void)hipFree(c->tw.lam); }
   (void)hipHostFree(c->host_out);
   delete c;
   return pc::PC_OK;
@@ -1443,8 +1468,15 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
       if (tri) {
         hipLaunchKernelGGL(tri_bisect, dim3((k + 3) / 4), dim3(256), 0, c->stream, c->out, c->E, k, c->tw);
         PC_LAUNCH_CHECK();
-        hipLaunchKernelGGL(tri_invit, dim3((k + kInvT - 1) / kInvT), dim3(kInvT), 0, c->stream, c->out, c->E, k,
-                           c->tw);
+        {
+          const int nt = tri_invit_threads(k);
+          if (nt == 16)
+            hipLaunchKernelGGL(tri_invit<16>, dim3((k + 15) / 16), dim3(16), tri_invit_lds(k), c->stream, c->out, c->E,
+                               k, c->tw);
+          else
+            hipLaunchKernelGGL(tri_invit<8>, dim3((k + 7) / 8), dim3(8), tri_invit_lds(k), c->stream, c->out, c->E,
+                               k, c->tw);
+        }
         PC_LAUNCH_CHECK();
         hipLaunchKernelGGL(tri_cluster, dim3(k), dim3(256), (size_t)kTriMaxM * k * sizeof(double), c->stream, c->out,
                            c->E, k, c->tw, c->tmp, c->tri_lam, c->info + 2);
