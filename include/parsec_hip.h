@@ -113,8 +113,8 @@ int pc_dense_syevd(void* ctx, int k, double* A, int lda, double* w_host);
  * 1 = Jacobi (syevj), 2 = one-launch Householder tridiagonalization + the
  * tridiagonal eigensolver of this library (multisection bisection, inverse
  * iteration, Gram-Schmidt and Rayleigh-Ritz within clusters of close
- * eigenvalues; rocSOLVER stedc for a cluster of more than 32) + ormtr
- * (default; 32 workgroups exchanging through agent-scope atomics for
+ * eigenvalues; rocSOLVER stedc for a cluster of more than 32) + Q Z
+ * (pc_dense_set_backtransform) (default; 32 workgroups exchanging through agent-scope atomics for
  * 128 <= k <= 320, one workgroup up to 512, larger k uses syevd), 3 = as 2
  * with the one-workgroup reduction always, 4 = as 2 with stedc always */
 int pc_dense_set_method(void* ctx, int method);
@@ -136,6 +136,11 @@ int pc_dense_gram_chol_async(void* ctx, int fp32, const void* Y, long long ldy, 
  * Cholesky factor and inverse (two launches) + GEMMs (default; k > 320 uses 0).
  * No counterpart in the reference: its QR gives W directly. */
 int pc_dense_set_triangular(void* ctx, int mode);
+/* how pc_dense_rr (methods 2-4) applies the tridiagonalization's reflectors to
+ * the tridiagonal eigenvectors, Q Z: 1 = this library's compact-WY kernels
+ * (default; k > 320 uses 0), 0 = rocSOLVER's ormtr.  No counterpart in the
+ * reference (its eigh is scipy's). */
+int pc_dense_set_backtransform(void* ctx, int mode);
 /* steps repeated with stedc because a cluster was too large (no counterpart
  * in the reference; for tests) */
 int pc_dense_tri_fallbacks(void* ctx, int* out);

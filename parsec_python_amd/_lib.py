@@ -63,6 +63,7 @@ PROTOTYPES = {
     "pc_dense_set_method": [_vp, _i],
     "pc_dense_tri_fallbacks": [_vp, _pi],
     "pc_dense_set_triangular": [_vp, _i],
+    "pc_dense_set_backtransform": [_vp, _i],
     "pc_dense_chol_async": [_vp, _i, _vp, _i],
     "pc_dense_gram_chol_async": [_vp, _i, _vp, _ll, _ll, _i, _vp, _i, _vp, _sz],
     "pc_dense_shift_count": [_vp, C.POINTER(_i)],

@@ -42,7 +42,7 @@ struct DenseCtx {
   double* out;      // kmax + 4 : eigenvalues then 3 info codes (as doubles)
   int* info;        // 4
   double* host_out; // pinned, kmax + 4
-  int method;       // 0: syevd, 1: Jacobi (syevj), 2: one-launch tridiagonalization + tri_* + ormtr
+  int method;       // 0: syevd, 1: Jacobi (syevj), 2: one-launch tridiagonalization + tri_* + Q Z
   double* tau;      // kmax (method 2: reflector scalars)
   double* P;        // method 2: padded copy of G, 64*ceil(kmax/64) x (kmax+1)
   double* resid;    // syevj: off-diagonal norm at exit
@@ -75,6 +75,8 @@ struct DenseCtx {
   double* cRi;
   double* cD;
   int tri_fallbacks;     // steps repeated with stedc
+  int orm_mode;          // Q Z: 1 = orm_tmat + orm_apply (default, k <= kOrmMaxK), 0 = rocSOLVER's ormtr
+  double* ormT;          // orm_tmat's triangular factors, kOrmNB^2 per block of reflectors
 };
 
 // S += shift * max_i S_ii on the diagonal (one workgroup of 256 threads)
@@ -1237,6 +1239,175 @@ __global__ __launch_bounds__(256) void tri_cluster(const double* __restrict__ d,
     }                                                                                        \
   } while (0)
 
+// ---- back-transformation Q Z of the tridiagonal eigenvectors (k <= 320) ----
+//
+// Q = H(0) H(1) .. H(k-2) from the tridiagonalization above (reflector i:
+// v_i(i+1) = 1, v_i(i+2:) in P(i+2:, i), scalar tau_i).  The reflectors go in
+// blocks of kOrmNB, B_b = H(i0) .. H(i0 + kOrmNB - 1) = I - V_b T_b V_b^T with
+// T_b upper triangular (the compact WY form of LAPACK's dlarft, forward and
+// columnwise: T(t,t) = tau_t, T(0:t,t) = -tau_t T(0:t,0:t) V(:,0:t)^T v_t), and
+// Q Z = B_0 (B_1 (.. (B_last Z))).  orm_tmat builds every T_b, one workgroup
+// per block: the Gram matrix V_b^T V_b on MFMA, then row s of T_b by lane s
+// (T(s,t) needs only T(s,q < t) and the Gram matrix, so the rows need no
+// barriers).  orm_apply holds kOrmNC columns of Z in LDS per
+// workgroup and applies the blocks last to first, W = V_b^T Z, W = T_b W,
+// Z -= V_b W, each product on MFMA (v_mfma_f64_16x16x4f64, the lane layout of
+// the Cholesky kernels above), with V_b staged through LDS.  Two launches in
+// place of rocSOLVER's ormtr, whose blocked path launches about thirty small
+// kernels (0.5 ms at k = 275).
+constexpr int kOrmNB = 32;
+constexpr int kOrmNC = 16;
+constexpr int kOrmMaxK = 320;
+constexpr int kOrmLdv = kOrmMaxK | 1;   // LDS stride of a staged V column
+
+// Vs[t * kOrmLdv + rr] = v_{i0+t}(i0 + 1 + rr) for rr < k - i0 - 1 (zero
+// columns past the last reflector).  A thread loads all kOrmNB entries of its
+// row before storing any (the column index is clamped to a reflector that
+// exists, the value selected afterwards), so its loads are in flight together.
+__device__ __forceinline__ void orm_stage_v(const double* __restrict__ P, int ldp, int k, int i0, int nr,
+                                            double* Vs) {
+  const int L = k - i0 - 1;
+  for (int rr = threadIdx.x; rr < L; rr += blockDim.x) {
+    const double* src = P + (long long)i0 * ldp + i0 + 1 + rr;
+    double v[kOrmNB];
+#pragma unroll
+    for (int t = 0; t < kOrmNB; ++t) {
+      v[t] = *src;
+      if (t + 1 < nr) src += ldp;
+    }
+#pragma unroll
+    for (int t = 0; t < kOrmNB; ++t) Vs[t * kOrmLdv + rr] = t < nr && rr >= t ? (rr > t ? v[t] : 1.0) : 0.0;
+  }
+}
+
+__device__ __forceinline__ v4d orm_mfma(double a, double b, v4d acc) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+}
+
+__global__ __launch_bounds__(256) void orm_tmat(const double* __restrict__ P, int ldp, int k,
+                                                const double* __restrict__ tau, double* __restrict__ Tw) {
+  __shared__ double Vs[kOrmNB * kOrmLdv];
+  __shared__ double g[kOrmNB][kOrmNB + 1];
+  __shared__ double Tl[kOrmNB][kOrmNB + 1];
+  const int b = blockIdx.x, i0 = b * kOrmNB;
+  const int nr = min(kOrmNB, k - 1 - i0);
+  const int L = k - i0 - 1;
+  orm_stage_v(P, ldp, k, i0, nr, Vs);
+  __syncthreads();
+  // g = V^T V: waves 0, 1, 2 take the 16 x 16 tiles (0,0), (0,16), (16,16)
+  const int l = threadIdx.x & 63, w = threadIdx.x >> 6, lm = l & 15, lk = l >> 4;
+  if (w < 3) {
+    const int s0 = w == 2 ? 16 : 0, t0 = w == 0 ? 0 : 16;
+    const double* va = Vs + (s0 + lm) * kOrmLdv;
+    const double* vb = Vs + (t0 + lm) * kOrmLdv;
+    v4d acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+    int rr = 0;
+    for (; rr + 8 <= L; rr += 8) {
+      acc0 = orm_mfma(va[rr + lk], vb[rr + lk], acc0);
+      acc1 = orm_mfma(va[rr + 4 + lk], vb[rr + 4 + lk], acc1);
+    }
+    for (; rr < L; rr += 4) {
+      const int r = rr + lk;
+      acc0 = orm_mfma(r < L ? va[r] : 0.0, r < L ? vb[r] : 0.0, acc0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) g[s0 + 4 * r + lk][t0 + lm] = acc0[r] + acc1[r];
+  }
+  __syncthreads();
+  // row s of T_b by lane s, kept in LDS (only lane s reads it back: no
+  // barriers between the columns)
+  if (threadIdx.x < kOrmNB) {
+    const int s = threadIdx.x;
+    for (int t = 0; t < kOrmNB; ++t) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+      int q = s;   // T(s,q) = 0 for q < s
+      for (; q + 4 <= t; q += 4) {
+        a0 += Tl[s][q] * g[q][t];
+        a1 += Tl[s][q + 1] * g[q + 1][t];
+        a2 += Tl[s][q + 2] * g[q + 2][t];
+        a3 += Tl[s][q + 3] * g[q + 3][t];
+      }
+      for (; q < t; ++q) a0 += Tl[s][q] * g[q][t];
+      const double tt = t < nr ? tau[i0 + t] : 0.0;
+      Tl[s][t] = s < t ? -tt * ((a0 + a1) + (a2 + a3)) : (s == t ? tt : 0.0);
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < kOrmNB * kOrmNB; idx += blockDim.x)
+    Tw[(long long)b * kOrmNB * kOrmNB + idx] = Tl[idx / kOrmNB][idx % kOrmNB];
+}
+
+__global__ __launch_bounds__(256) void orm_apply(const double* __restrict__ P, int ldp, int k,
+                                                 const double* __restrict__ Tw, double* __restrict__ Z, int ldz) {
+  __shared__ double Vs[kOrmNB * kOrmLdv];
+  __shared__ double Zs[kOrmMaxK * kOrmNC];            // Zs[r * kOrmNC + c]
+  __shared__ double Ts[kOrmNB][kOrmNB + 1];
+  __shared__ double Wp[2][kOrmNB][kOrmNC + 1];        // V^T Z over the two halves of the rows
+  __shared__ double W2[kOrmNB][kOrmNC + 1];
+  const int c0 = blockIdx.x * kOrmNC;
+  const int nc = min(kOrmNC, k - c0);
+  const int tid = threadIdx.x, l = tid & 63, w = tid >> 6, lm = l & 15, lk = l >> 4;
+  for (int c = 0; c < kOrmNC; ++c)
+    for (int r = tid; r < k; r += blockDim.x) Zs[r * kOrmNC + c] = c < nc ? Z[(long long)(c0 + c) * ldz + r] : 0.0;
+  const int nbk = (k - 1 + kOrmNB - 1) / kOrmNB;
+  for (int b = nbk - 1; b >= 0; --b) {
+    const int i0 = b * kOrmNB, nr = min(kOrmNB, k - 1 - i0), L = k - i0 - 1;
+    __syncthreads();   // the previous block is done with Vs, Ts and Zs
+    orm_stage_v(P, ldp, k, i0, nr, Vs);
+    for (int idx = tid; idx < kOrmNB * kOrmNB; idx += blockDim.x)
+      Ts[idx / kOrmNB][idx % kOrmNB] = Tw[(long long)b * kOrmNB * kOrmNB + idx];
+    __syncthreads();
+    {
+      // W = V^T Z (rows i0 + 1 .. k - 1): wave w takes rows t0 = 16 (w & 1) of
+      // W over half h = w >> 1 of the rows of Z
+      const int t0 = 16 * (w & 1), h = w >> 1;
+      const int steps = (L + 3) / 4, half = 4 * ((steps + 1) / 2);
+      const int rb = h ? half : 0, re = h ? 4 * steps : half;
+      const double* va = Vs + (t0 + lm) * kOrmLdv;
+      const double* zb = Zs + (i0 + 1) * kOrmNC + lm;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+      for (int rr = rb; rr < re; rr += 4) {
+        const int r = rr + lk;
+        acc = orm_mfma(r < L ? va[r] : 0.0, r < L ? zb[r * kOrmNC] : 0.0, acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) Wp[h][t0 + 4 * r + lk][lm] = acc[r];
+    }
+    __syncthreads();
+    if (w < 2) {
+      // W2 = T W (rows t0 = 16 w)
+      const int t0 = 16 * w;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q0 = 0; q0 < kOrmNB; q0 += 4) {
+        const int q = q0 + lk;
+        acc = orm_mfma(Ts[t0 + lm][q], Wp[0][q][lm] + Wp[1][q][lm], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) W2[t0 + 4 * r + lk][lm] = acc[r];
+    }
+    __syncthreads();
+    // Z -= V W2, tiles of 16 rows over the waves
+    for (int rr0 = 16 * w; rr0 < L; rr0 += 64) {
+      const int ra = rr0 + lm;
+      v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int q0 = 0; q0 < kOrmNB; q0 += 4) {
+        const int q = q0 + lk;
+        acc = orm_mfma(ra < L ? Vs[q * kOrmLdv + ra] : 0.0, W2[q][lm], acc);
+      }
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int rr = rr0 + 4 * r + lk;
+        if (rr < L) Zs[(i0 + 1 + rr) * kOrmNC + lm] -= acc[r];
+      }
+    }
+  }
+  __syncthreads();
+  for (int c = 0; c < nc; ++c)
+    for (int r = tid; r < k; r += blockDim.x) Z[(long long)(c0 + c) * ldz + r] = Zs[r * kOrmNC + c];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1268,6 +1439,8 @@ int pc_dense_create(void* stream, int kmax, void** out_ctx) {
   PC_HIP(hipMalloc(&c->pub, 4 * kTrdMwgMax * sizeof(double)));
   c->tri_mode = 2;
   c->blk_fallbacks = 0;
+  c->orm_mode = 1;
+  PC_HIP(hipMalloc(&c->ormT, (size_t)((kOrmMaxK - 1 + kOrmNB - 1) / kOrmNB) * kOrmNB * kOrmNB * sizeof(double)));
   PC_HIP(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
   PC_HIP(hipEventCreateWithFlags(&c->ev_s, hipEventDisableTiming));
   PC_HIP(hipEventCreateWithFlags(&c->ev_c, hipEventDisableTiming));
@@ -1332,7 +1505,7 @@ int pc_dense_destroy(void* ctx) {
   (void)hipFree(c->tmp); (void)hipFree(c->E); (void)hipFree(c->out); (void)hipFree(c->info);
   (void)hipFree(c->resid); (void)hipFree(c->sweeps); (void)hipFree(c->tau); (void)hipFree(c->P);
   (void)hipFree(c->S0); (void)hipFree(c->G0);
-  (void)hipFree(c->cR); (void)hipFree(c->cRi); (void)hipFree(c->cD);
+  (void)hipFree(c->cR); (void)hipFree(c->cRi); (void)hipFree(c->cD); (void)hipFree(c->ormT);
   if (c->tw.x) { (void)hipFree(c->tw.x); (void)hipFree(c->tw.lam); }
   (void)hipHostFree(c->host_out);
   delete c;
@@ -1485,8 +1658,19 @@ int pc_dense_rr(void* ctx, int k, double* S, int lds, double* Gy, int ldg, doubl
       } else {
         PC_RB(rocsolver_dstedc(c->h, rocblas_evect_tridiagonal, k, c->out, c->E, c->tmp, k, c->info + 2));
       }
-      PC_RB(rocsolver_dormtr(c->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, k, k, c->P,
-                             64 * nch, c->tau, c->tmp, k));
+      if (c->orm_mode == 1 && k <= kOrmMaxK) {
+        if (k > 1) {   // (k = 1: no reflectors, Q = I)
+          hipLaunchKernelGGL(orm_tmat, dim3((k - 1 + kOrmNB - 1) / kOrmNB), dim3(256), 0, c->stream, c->P, 64 * nch,
+                             k, c->tau, c->ormT);
+          PC_LAUNCH_CHECK();
+          hipLaunchKernelGGL(orm_apply, dim3((k + kOrmNC - 1) / kOrmNC), dim3(256), 0, c->stream, c->P, 64 * nch, k,
+                             c->ormT, c->tmp, k);
+          PC_LAUNCH_CHECK();
+        }
+      } else {
+        PC_RB(rocsolver_dormtr(c->h, rocblas_side_left, rocblas_fill_lower, rocblas_operation_none, k, k, c->P,
+                               64 * nch, c->tau, c->tmp, k));
+      }
       Q = c->tmp;
       ldq = k;
     } else if (c->method == 1) {
@@ -1669,6 +1853,14 @@ int pc_dense_set_triangular(void* ctx, int mode) {
   PC_CHECK_ARG(c, "null context");
   PC_CHECK_ARG(mode >= 0 && mode <= 2, "mode must be 0 (trtri + trmm), 1 (trsm) or 2 (blocked Cholesky + GEMM)");
   c->tri_mode = mode;
+  return pc::PC_OK;
+}
+
+int pc_dense_set_backtransform(void* ctx, int mode) {
+  DenseCtx* c = static_cast<DenseCtx*>(ctx);
+  PC_CHECK_ARG(c, "null context");
+  PC_CHECK_ARG(mode == 0 || mode == 1, "mode must be 0 (rocSOLVER ormtr) or 1 (orm_tmat + orm_apply)");
+  c->orm_mode = mode;
   return pc::PC_OK;
 }
 

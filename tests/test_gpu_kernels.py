@@ -251,9 +251,9 @@ def test_dense_rr_generalized_eig(gpu):
 
 
 @pytest.mark.parametrize("method", [0, 1, 2, 3, 4])
-@pytest.mark.parametrize("k,kind", [(1, "rand"), (2, "rand"), (3, "rand"), (65, "rand"), (128, "rand"),
-                                    (275, "degenerate"), (275, "band"), (300, "rand"), (320, "degenerate"),
-                                    (520, "rand")])
+@pytest.mark.parametrize("k,kind", [(1, "rand"), (2, "rand"), (3, "rand"), (33, "rand"), (65, "rand"),
+                                    (128, "rand"), (275, "degenerate"), (275, "band"), (300, "rand"),
+                                    (320, "degenerate"), (400, "rand"), (520, "rand")])
 def test_dense_rr_methods(gpu, method, k, kind):
     """Every eigensolver of the dense stage (syevd, syevj, tridiagonalization
     with 32 workgroups (methods 2 and 4, 128 <= k <= 320) or one (method 3),
@@ -291,9 +291,14 @@ def test_dense_rr_methods(gpu, method, k, kind):
     scale = max(1.0, np.abs(ref).max())
     # R^-1 from the blocked Cholesky kernels (the default; rocSOLVER's potrf
     # + trtri above k = 320), and (method 2) also as trtri + trmm and as
-    # triangular solves
-    for tri_mode in ((2, 0, 1) if method == 2 else (2,)):
+    # triangular solves; Q Z (methods 2-4) by this library's kernels (the
+    # default, k <= 320) and by rocSOLVER's ormtr
+    combos = [(2, 1), (0, 1), (1, 1)] if method == 2 else [(2, 1)]
+    if method in (2, 3, 4):
+        combos.append((2, 0))
+    for tri_mode, orm in combos:
         _lib.call("pc_dense_set_triangular", la.dense, tri_mode)
+        _lib.call("pc_dense_set_backtransform", la.dense, orm)
         Sd = torch.tensor(S.T.copy(), device="cuda").view(-1)
         Gd = torch.tensor(G.T.copy(), device="cuda").view(-1)
         T, lam = la.dense_rr(Sd, Gd, k)
