@@ -380,9 +380,11 @@ def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
     """Time of one Rayleigh-Ritz dense stage at k = 275 with this library's
     tridiagonal eigensolver (method 2) and with rocSOLVER's stedc (method 4),
     with R^-1 from rocSOLVER's potrf + trtri, from triangular solves and from
-    the blocked Cholesky kernels; all give the same pairs.  Printed for the
-    record; the bars are that the library's eigensolver is not slower than
-    stedc and the blocked Cholesky kernels not slower than potrf + trtri."""
+    the blocked Cholesky kernels, and Q Z by this library's kernels or by
+    rocSOLVER's ormtr; all give the same pairs.  Printed for the record; the
+    bars are that the library's eigensolver is not slower than stedc, the
+    blocked Cholesky kernels not slower than potrf + trtri and the library's
+    Q Z not slower than ormtr."""
     import time
     import torch
     from parsec_python_amd import _lib
@@ -396,10 +398,11 @@ def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
     Sd0 = torch.tensor(S.T.copy(), device="cuda").view(-1)
     Gd0 = torch.tensor(G.T.copy(), device="cuda").view(-1)
     out = {}
-    for method, tri_mode in ((2, 0), (4, 0), (2, 1), (2, 2)):
+    for method, tri_mode, orm in ((2, 0, 1), (4, 0, 1), (2, 1, 1), (2, 2, 1), (2, 2, 0)):
         la = BlockLinalg(10, k)
         _lib.call("pc_dense_set_method", la.dense, method)
         _lib.call("pc_dense_set_triangular", la.dense, tri_mode)
+        _lib.call("pc_dense_set_backtransform", la.dense, orm)
         ts = []
         for rep in range(6):
             Sd, Gd = Sd0.clone(), Gd0.clone()
@@ -407,15 +410,16 @@ def test_dense_rr_timing_tridiagonal_eigensolvers(gpu):
             t0 = time.perf_counter()
             T, lam = la.dense_rr(Sd, Gd, k)
             ts.append(time.perf_counter() - t0)
-        out[(method, tri_mode)] = (float(np.median(ts[1:])) * 1e3, lam)
+        out[(method, tri_mode) if orm else "ormtr"] = (float(np.median(ts[1:])) * 1e3, lam)
         la.close()
     print(f"  dense stage k {k}: tridiagonal eigensolver {out[(2, 0)][0]:.2f} ms, stedc {out[(4, 0)][0]:.2f} ms "
           f"(potrf + trtri); with triangular solves {out[(2, 1)][0]:.2f} ms; with the blocked Cholesky kernels "
-          f"{out[(2, 2)][0]:.2f} ms", flush=True)
-    for key in ((2, 0), (2, 1), (2, 2)):
+          f"{out[(2, 2)][0]:.2f} ms, and with rocSOLVER's ormtr for Q Z {out['ormtr'][0]:.2f} ms", flush=True)
+    for key in ((2, 0), (2, 1), (2, 2), "ormtr"):
         assert np.allclose(out[key][1], out[(4, 0)][1], rtol=0, atol=1e-12 * k * np.abs(out[(4, 0)][1]).max())
     assert out[(2, 0)][0] <= 1.05 * out[(4, 0)][0]
     assert out[(2, 2)][0] <= 1.05 * out[(2, 0)][0]
+    assert out[(2, 2)][0] <= 1.05 * out["ormtr"][0]
 
 
 @pytest.mark.parametrize("k", [34, 275])
