@@ -182,6 +182,15 @@ int pc_dots(long long n, const double* A, long long lda, int m, const double* v,
             void* stream);
 int pc_lincomb(long long n, const double* A, long long lda, int m, const double* coef_host, double alpha,
                const double* x, double beta, double* y, void* stream);
+/* pc_lincomb with its scalars in device memory (pc_dots results): coef[i] =
+ * coef_scale * coef_dev[i]; alpha = 1 / alpha_recip_dev[0] (0 when
+ * |v| < alpha_tol |v|) when alpha_recip_dev is set; beta = beta_scale *
+ * beta_dev[0] when beta_dev is set.  Lets the multisecant mixer
+ * (src/Mixer/msecant1.py:40-175, whose numpy dots feed the next update
+ * directly) run its update with one host readback instead of seven. */
+int pc_lincomb_dev(long long n, const double* A, long long lda, int m, const double* coef_dev, double coef_scale,
+                   double alpha, const double* alpha_recip_dev, double alpha_tol, const double* x, double beta,
+                   const double* beta_dev, double beta_scale, double* y, void* stream);
 /* Lanczos bound without host round trips (replaces the vdot / norm / axpy
  * sequence of src/Eigensolvers/lanczosForChefsi1.py:5-99): pc_lanczos_start
  * normalises v; after w = H v1, pc_lanczos_step(k) computes alpha_k = v1.w,

@@ -77,6 +77,7 @@ PROTOTYPES = {
     "pc_dots_nchunk": [_ll, _pi],
     "pc_dots": [_ll, _vp, _ll, _i, _vp, _vp, _vp, _vp],
     "pc_lincomb": [_ll, _vp, _ll, _i, _pd, _d, _vp, _d, _vp, _vp],
+    "pc_lincomb_dev": [_ll, _vp, _ll, _i, _vp, _d, _d, _vp, _d, _vp, _d, _vp, _d, _vp, _vp],
     "pc_lanczos_nblocks": [_ll, _pi],
     "pc_lanczos_start": [_ll, _vp, _vp, _vp],
     "pc_lanczos_step": [_ll, _vp, _vp, _vp, _vp, _i, _vp, _vp],

@@ -152,6 +152,34 @@ __global__ __launch_bounds__(kBlk) void lincomb_kernel(long long n, const double
   y[p] = s;
 }
 
+// The same combination with its scalars read from device memory (results of
+// pc_dots, so a caller needs no host round trip): coef[i] = cs * cdev[i];
+// alpha = 1 / adev[0] (0 when |adev[0]| < atol |adev[0]|) when adev is set;
+// beta = bs * bdev[0] when bdev is set.  Same arithmetic as lincomb_kernel.
+__global__ __launch_bounds__(kBlk) void lincomb_dev_kernel(long long n, const double* __restrict__ A, long long lda,
+                                                           int m, const double* __restrict__ cdev, double cs,
+                                                           double alpha, const double* __restrict__ adev, double atol,
+                                                           const double* __restrict__ x, double beta,
+                                                           const double* __restrict__ bdev, double bs,
+                                                           double* __restrict__ y) {
+  __shared__ double cf[64];
+  if ((int)threadIdx.x < m) cf[threadIdx.x] = cs * cdev[threadIdx.x];
+  __syncthreads();
+  const long long p = (long long)blockIdx.x * kBlk + threadIdx.x;
+  if (p >= n) return;
+  double al = alpha;
+  if (adev) {
+    const double v = adev[0];
+    al = fabs(v) < atol * fabs(v) ? 0.0 : 1.0 / v;
+  }
+  const double be = bdev ? bs * bdev[0] : beta;
+  double s = al * y[p];
+  if (x) s += be * x[p];
+#pragma unroll 8
+  for (int i = 0; i < m; ++i) s += cf[i] * A[(long long)i * lda + p];
+  y[p] = s;
+}
+
 // ---------------------------------------------------------------- conversion
 template <typename A, typename B>
 __global__ __launch_bounds__(kBlk) void convert_kernel(long long n, int ncol, const A* __restrict__ src, long long lds,
@@ -365,6 +393,21 @@ int pc_lincomb(long long n, const double* A, long long lda, int m, const double*
   for (int i = 0; i < 64; ++i) cf.c[i] = i < m ? coef_host[i] : 0.0;
   hipLaunchKernelGGL(lincomb_kernel, dim3(blocks(n)), dim3(kBlk), 0, (hipStream_t)stream, n, A, lda, m, cf, alpha, x,
                      beta, y);
+  PC_LAUNCH_CHECK();
+  return pc::PC_OK;
+}
+
+// y = alpha*y + beta*x + sum_{i<m} coef[i] A_i with device scalars (see
+// lincomb_dev_kernel); coef_dev may be null when m = 0, alpha_recip_dev and
+// beta_dev null to use alpha and beta as given.
+int pc_lincomb_dev(long long n, const double* A, long long lda, int m, const double* coef_dev, double coef_scale,
+                   double alpha, const double* alpha_recip_dev, double alpha_tol, const double* x, double beta,
+                   const double* beta_dev, double beta_scale, double* y, void* stream) {
+  PC_CHECK_ARG(n >= 1 && y && m >= 0 && m <= 64, "bad argument");
+  PC_CHECK_ARG(m == 0 || (A && coef_dev && lda >= n), "bad matrix argument");
+  PC_CHECK_ARG(!beta_dev || x, "beta_dev without x");
+  hipLaunchKernelGGL(lincomb_dev_kernel, dim3(blocks(n)), dim3(kBlk), 0, (hipStream_t)stream, n, A, lda, m, coef_dev,
+                     coef_scale, alpha, alpha_recip_dev, alpha_tol, x, beta, beta_dev, beta_scale, y);
   PC_LAUNCH_CHECK();
   return pc::PC_OK;
 }

@@ -193,6 +193,24 @@ class VecOps:
         _lib.call("pc_dots", self.n, ptr(A), lda, m, ptr(v), ptr(part), ptr(out), stream())
         return out
 
+    def lincomb_dev(self, y: torch.Tensor, alpha: float = 1.0, x: torch.Tensor | None = None, beta: float = 0.0,
+                    A: torch.Tensor | None = None, coef: torch.Tensor | None = None, coef_scale: float = 1.0,
+                    alpha_recip: torch.Tensor | None = None, alpha_tol: float = 0.0,
+                    beta_dev: torch.Tensor | None = None, beta_scale: float = 1.0) -> None:
+        """lincomb with its scalars on the device (pc_dots results, no host
+        round trip): coefficients coef_scale * coef, alpha = 1 / alpha_recip[0]
+        (0 when |v| < alpha_tol |v|) if given, beta = beta_scale * beta_dev[0]
+        if given."""
+        m = 0 if coef is None else int(coef.numel())
+        if m > 64:
+            raise ValueError("lincomb supports at most 64 columns per call")
+        nul = C.c_void_p(0)
+        _lib.call("pc_lincomb_dev", self.n, ptr(A) if A is not None else nul,
+                  (A.shape[-1] if A is not None else self.n), m, ptr(coef) if coef is not None else nul, coef_scale,
+                  alpha, ptr(alpha_recip) if alpha_recip is not None else nul, alpha_tol,
+                  ptr(x) if x is not None else nul, beta, ptr(beta_dev) if beta_dev is not None else nul,
+                  beta_scale, ptr(y), stream())
+
     def lincomb(self, y: torch.Tensor, alpha: float = 1.0, x: torch.Tensor | None = None, beta: float = 0.0,
                 A: torch.Tensor | None = None, coef=None, lda: int | None = None) -> None:
         import numpy as np

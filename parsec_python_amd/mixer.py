@@ -23,7 +23,11 @@ from .hamiltonian import VecOps
 
 class DeviceMsecant1:
     def __init__(self, n: int, capacity: int, mix: float = 0.5, restart_factor: float = 0.1,
-                 tol: float = float(np.finfo(float).eps)):
+                 tol: float = float(np.finfo(float).eps), device_scalars: bool = True):
+        # device_scalars: the dot products feed the updates on the device
+        # (pc_lincomb_dev), one host readback per call (the restart test)
+        # instead of seven; the same arithmetic, so the same bits
+        self.device_scalars = device_scalars
         self.n = n
         self.cap = capacity
         self.mix = mix
@@ -54,8 +58,12 @@ class DeviceMsecant1:
             vec.lincomb(out, 1.0, f1, mix)
             return 0
         if m >= 2:
-            nprev = math.sqrt(float(self._dots(DF[m - 1:m], 1, DF[m - 1])[0]))
-            nf = math.sqrt(float(self._dots(f1.view(1, -1), 1, f1)[0]))
+            if self.device_scalars:
+                nn = torch.cat([vec.dots(DF[m - 1:m], 1, DF[m - 1]), vec.dots(f1.view(1, -1), 1, f1)]).cpu().numpy()
+                nprev, nf = math.sqrt(float(nn[0])), math.sqrt(float(nn[1]))
+            else:
+                nprev = math.sqrt(float(self._dots(DF[m - 1:m], 1, DF[m - 1])[0]))
+                nf = math.sqrt(float(self._dots(f1.view(1, -1), 1, f1)[0]))
             if nprev < self.restart_factor * nf:
                 DX[0].copy_(DX[m - 1])
                 DF[0].copy_(DF[m - 1])
@@ -65,6 +73,8 @@ class DeviceMsecant1:
                 return 0
         if m + 1 > self.cap:
             raise RuntimeError("mixer capacity exceeded")
+        if self.device_scalars:
+            return self._update_on_device(x1, f1, out)
         ng = m - 1
         last_x, last_f = DX[m - 1], DF[m - 1]
         vec.lincomb(last_x, -1.0, x1, 1.0)          # dx = x1 - x_prev
@@ -95,6 +105,42 @@ class DeviceMsecant1:
         vec.lincomb(last_f, cinv)
         d = float(self._dots(last_f.view(1, -1), 1, f1)[0])
         vec.lincomb(out, 1.0, last_x, -d)
+        DX[m].copy_(x1)
+        DF[m].copy_(f1)
+        self.m = m + 1
+        return m
+
+    def _update_on_device(self, x1: torch.Tensor, f1: torch.Tensor, out: torch.Tensor) -> int:
+        """The secant update of __call__ with every dot product kept on the
+        device (src/Mixer/msecant1.py:40-175): the same steps and arithmetic."""
+        mix, m = self.mix, self.m
+        DX, DF, vec = self.DX, self.DF, self.vec
+        ng = m - 1
+        last_x, last_f = DX[m - 1], DF[m - 1]
+        vec.lincomb(last_x, -1.0, x1, 1.0)          # dx = x1 - x_prev
+        vec.lincomb(last_f, -1.0, f1, 1.0)          # df = f1 - f_prev
+        dx = self.tmp[0]
+        dx.copy_(last_x)
+        if ng > 0:
+            vec.lincomb_dev(last_x, 1.0, last_f, mix, A=DX, coef=vec.dots(DF, ng, last_f), coef_scale=-1.0)
+        else:
+            vec.lincomb(last_x, 1.0, last_f, mix)
+        out.copy_(x1)
+        if ng > 0:
+            vec.lincomb_dev(out, 1.0, f1, mix, A=DX, coef=vec.dots(DF, ng, f1), coef_scale=-1.0)
+        else:
+            vec.lincomb(out, 1.0, f1, mix)
+        nvec = self.tmp[1]
+        nvec.zero_()
+        if ng > 0:
+            vec.lincomb_dev(nvec, 0.0, dx, -mix, A=DF, coef=vec.dots(DX, ng, dx))
+        else:
+            vec.lincomb(nvec, 0.0, dx, -mix)
+        mval = vec.dots(nvec.view(1, -1), 1, last_f)
+        last_f.copy_(nvec)
+        vec.lincomb_dev(last_f, alpha_recip=mval, alpha_tol=self.tol)   # last_f = nvec / mval
+        d = vec.dots(last_f.view(1, -1), 1, f1)
+        vec.lincomb_dev(out, 1.0, last_x, beta_dev=d, beta_scale=-1.0)
         DX[m].copy_(x1)
         DF[m].copy_(f1)
         self.m = m + 1

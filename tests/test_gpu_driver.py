@@ -231,6 +231,29 @@ def test_device_mixers_match_the_oracle(gpu, name):
     assert np.linalg.norm(F(x)) < 1e-3 * np.linalg.norm(F(np.random.default_rng(12).standard_normal(n)))
 
 
+def test_msecant1_device_scalars_give_the_same_bits(gpu):
+    """The multisecant-1 update with its dot products kept on the device
+    (pc_lincomb_dev, one host readback per call) against the same update with
+    every scalar read back to the host: identical outputs at every step of a
+    nonlinear fixed-point sequence, restarts included."""
+    import torch
+    from parsec_python_amd.mixer import DeviceMsecant1
+    n, steps = 30011, 16
+    F = _residual_map(n, 11)
+    dev = DeviceMsecant1(n, steps + 2, device_scalars=True)
+    host = DeviceMsecant1(n, steps + 2, device_scalars=False)
+    x = np.random.default_rng(13).standard_normal(n)
+    out_d = torch.empty(n, dtype=torch.float64, device="cuda")
+    out_h = torch.empty(n, dtype=torch.float64, device="cuda")
+    for it in range(steps):
+        f = F(x)
+        md = dev(torch.tensor(x, device="cuda"), torch.tensor(f, device="cuda"), out_d)
+        mh = host(torch.tensor(x, device="cuda"), torch.tensor(f, device="cuda"), out_h)
+        assert md == mh, it
+        assert torch.equal(out_d, out_h), it
+        x = out_h.cpu().numpy()
+
+
 # ---------------------------------------------------------------------------- dense stage
 def test_dense_rr_shifted_cholesky_fallback(gpu):
     """An overlap matrix with a direction of negative curvature at 1e-12 of
