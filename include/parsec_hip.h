@@ -240,6 +240,13 @@ int pc_mg_destroy(void* mg);
 int pc_mg_workspace(void* mg, size_t* out_bytes);
 int pc_pcg_solve_mg(void* mg, const double* rhs, double* x, int maxit, double tol, int check_every, void* work,
                     size_t work_bytes, int* its_out, double* res_out, void* stream);
+/* iterations before pc_pcg_solve_mg first reads its state back (0, the
+ * default: its check_every); later readbacks follow every check_every.  A
+ * caller that knows about how many iterations a solve takes (the previous SCF
+ * iteration's) queues that many before the first host round trip instead of
+ * queueing past convergence.  Results do not depend on it.  No counterpart in
+ * the reference (its CG is a host loop). */
+int pc_mg_set_first_check(void* mg, int first);
 
 /* ---- ionic setup ------------------------------------------------------------------
  * replaces: the per-point spline loops of src/V_ion/pseudoDiag.py:199-278

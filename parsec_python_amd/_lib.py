@@ -93,6 +93,7 @@ PROTOTYPES = {
     "pc_mg_destroy": [_vp],
     "pc_mg_workspace": [_vp, _psz],
     "pc_pcg_solve_mg": [_vp, _vp, _vp, _i, _d, _i, _vp, _sz, _pi, _vp, _vp],
+    "pc_mg_set_first_check": [_vp, _i],
     "pc_sad_eval": [_vp, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "pc_spline_eval": [_vp, _vp, _vp, _vp, _i, _vp, _ll, _vp, _vp],
     # learned initialiser (3D U-Net) kernels

@@ -426,6 +426,7 @@ struct MGCtx {
   Tail tl;
   std::vector<void*> allocs;
   double* t0;
+  int first_check;            // iterations before the first state readback (0: check_every)
 };
 
 size_t work_bytes_for(long long n) {
@@ -493,6 +494,7 @@ extern "C" {
 int pc_mg_create(void* grid, double h, void** out_mg) {
   PC_CHECK_ARG(grid && out_mg && h > 0, "bad argument");
   MGCtx* c = new MGCtx();
+  c->first_check = 0;
   c->grid = grid;
   Fine& g = c->g;
   double w8[8];
@@ -602,6 +604,15 @@ int pc_mg_workspace(void* mg, size_t* out_bytes) {
   return pc::PC_OK;
 }
 
+// Iterations before the first state readback of pc_pcg_solve_mg (0: its
+// check_every); the results do not depend on it.
+int pc_mg_set_first_check(void* mg, int first) {
+  MGCtx* c = static_cast<MGCtx*>(mg);
+  PC_CHECK_ARG(c && first >= 0, "bad argument");
+  c->first_check = first;
+  return pc::PC_OK;
+}
+
 // Multigrid-preconditioned CG: same arguments and stop rule as pc_pcg_solve
 // (no first-iteration aliasing: this is the textbook preconditioned method).
 int pc_pcg_solve_mg(void* mg, const double* rhs, double* x, int maxit, double tol, int check_every, void* work,
@@ -645,8 +656,12 @@ int pc_pcg_solve_mg(void* mg, const double* rhs, double* x, int maxit, double to
   static thread_local double* host = nullptr;
   if (!host) PC_HIP(hipHostMalloc(&host, S_COUNT * sizeof(double), hipHostMallocDefault));
   int status = pc::PC_OK;
-  for (int t0 = 0; t0 <= maxit; t0 += check_every) {
-    for (int t = t0; t < t0 + check_every && t <= maxit; ++t) {
+  // the state is read back after iteration first_check (when set), then every
+  // check_every iterations; kernels queued past convergence return at once
+  int t = 0;
+  int next = c->first_check > 0 ? c->first_check : check_every;
+  while (t <= maxit) {
+    for (; t < next && t <= maxit; ++t) {
       // at t = 0 the direction is z itself: p_old = z with beta = 0
       const double* p_old = t == 0 ? z : p[t & 1];
       ops.apply(c, st, t, maxit, tol, z, p_old, p[(t + 1) & 1], ap, prr[t & 1], prz, ppq, sc);
@@ -669,6 +684,7 @@ int pc_pcg_solve_mg(void* mg, const double* rhs, double* x, int maxit, double to
       break;
     }
     if (host[S_DONE] != 0.0) break;
+    next = t + check_every;
   }
   if (!status) {
     if (host[S_DONE] == 0.0) {

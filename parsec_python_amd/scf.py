@@ -225,6 +225,7 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
     f1 = torch.empty(n, **dev)
     pot_next = torch.empty(n, **dev)
     occ2_dev = torch.empty(k, **dev)
+    occ2_pin = torch.empty(k, dtype=torch.float64, pin_memory=True)
 
     H.dtype = prob.dtype
     eig = Eigensolver(H, k, seed=cfg.seed, overlap=cfg.rr_overlap)
@@ -286,7 +287,8 @@ def run_scf(prob: SCFProblem, rho_init: torch.Tensor | None = None, hpot_init: t
             diag_label = f"chebsf_gpu (diagmeth={cfg.diagmeth})"
         t1 = time.perf_counter()
         _, occ = occupations(lam[:nev], cfg.fermi_temp, prob.nelec, 1e-6)
-        occ2_dev[:nev].copy_(torch.from_numpy(2.0 * np.asarray(occ, dtype=np.float64)))
+        occ2_pin[:nev] = torch.from_numpy(2.0 * np.asarray(occ, dtype=np.float64))
+        occ2_dev[:nev].copy_(occ2_pin[:nev], non_blocking=True)   # (the buffer is reused after a readback)
         sfx = "f64" if W.dtype == F64 else "f32"
         _lib.call(f"pc_density_{sfx}", ptr(W), W.shape[1], n, nev, ptr(occ2_dev), ptr(rho_cell), stream())
         _lib.call("pc_hartree_rhs", n, ptr(rho_cell), ptr(rho0), 4 * math.pi / h3, ptr(rhs), stream())

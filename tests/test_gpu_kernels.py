@@ -648,6 +648,36 @@ def test_pcg_multigrid(gpu, nx, ny, nz, kind, radius):
     dg.close()
 
 
+def test_pcg_multigrid_check_schedule_gives_the_same_bits(gpu):
+    """A warm solver (first state readback after about the previous solve's
+    iteration count, then every 2) and a fresh one (every check_every) give
+    the same iterations and the same solution bits: kernels queued past
+    convergence return at once."""
+    import torch
+    from parsec_python_amd.grid import DeviceGrid
+    from parsec_python_amd.hartree import HartreeSolver
+    h = 0.35
+    g = _grid(40, 40, 40, h, "sphere")
+    n = g.npts
+    rng = np.random.default_rng(41)
+    rhs1, rhs2 = (torch.tensor(rng.standard_normal(n), device="cuda") for _ in range(2))
+    dg = DeviceGrid(g, 4)
+    warm = HartreeSolver(dg, h, prec="mg")
+    fresh = HartreeSolver(dg, h, prec="mg")
+    warm.solve(rhs1, torch.zeros(n, dtype=torch.float64, device="cuda"), 500, 1e-5)
+    print(f"   first solve {warm.last_its} iterations", flush=True)
+    for tol in (1e-5, 1e-7):
+        xa = torch.zeros(n, dtype=torch.float64, device="cuda")
+        xb = torch.zeros(n, dtype=torch.float64, device="cuda")
+        ia = warm.solve(rhs2, xa, 500, tol)
+        ib = fresh.solve(rhs2, xb, 500, tol)
+        assert ia == ib
+        assert torch.equal(xa, xb)
+    warm.close()
+    fresh.close()
+    dg.close()
+
+
 @pytest.mark.parametrize("kind", ["cube", "sphere"])
 def test_sad_eval_vs_oracle(gpu, kind):
     from parsec_python_amd import ionic, molecules as M
